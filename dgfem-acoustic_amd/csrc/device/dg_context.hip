@@ -1,0 +1,898 @@
+// Context behind the C ABI: FP64 host precompute -> device-resident SoA, kernel dispatch,
+// the reference time loop (src/solver.cpp:311-584) driven in batches of steps between save
+// points, with per-step residual / observer records returned through device rings.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/dgfem_amd.h"
+#include "../host/mesh.hpp"
+#include "../host/refelem.hpp"
+#include "dg_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_createError;
+
+struct TableSource {
+    dg_value_fn fn;
+    void* user;
+};
+
+}  // namespace
+
+struct dg_context;
+using StageFn = void (*)(dg_context*, int stage, const void* qin, void* qout, void* uout, bool snap);
+using PerStepFn = void (*)(dg_context*);
+
+struct dg_context {
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int dim = 0, p = 0, Np = 0, Nfp = 0, Nfc = 0, nf = 4, precision = DG_FP64;
+    int64_t K = 0;
+    size_t tsize = 8;
+    dg::DGMesh mesh;
+    dg_physics phys{};
+    dg_options opt{};
+    // device state
+    void *u = nullptr, *ulo = nullptr, *qa = nullptr, *qb = nullptr, *acc = nullptr, *snap = nullptr,
+         *snaplo = nullptr, *geo = nullptr, *ops = nullptr;
+    int32_t *nbr = nullptr, *iops = nullptr;
+    uint8_t* code = nullptr;
+    double* resPartial = nullptr;
+    int64_t nblocks = 0;
+    // sources / observers (host definitions + device tables)
+    std::vector<std::vector<int64_t>> srcNodes;
+    std::vector<std::array<double, 5>> srcParams;  // kind, amp, freq, phase, duration
+    std::vector<TableSource> srcTables;            // per source (fn null for monopoles)
+    std::vector<std::vector<int64_t>> obsNodes;
+    std::vector<std::vector<double>> obsW;
+    int64_t *dSrcOff = nullptr, *dSrcIdx = nullptr, *dObsOff = nullptr, *dObsIdx = nullptr;
+    double *dSrcParams = nullptr, *dObsW = nullptr;
+    double *dT = nullptr, *dV = nullptr;
+    uint8_t* dVF = nullptr;
+    double *resRing = nullptr, *obsRing = nullptr;
+    int* nonfinite = nullptr;
+    int64_t* counter = nullptr;
+    int64_t ringSlots = 0;
+    // pinned staging
+    double *hT = nullptr, *hV = nullptr, *hRes = nullptr, *hObs = nullptr;
+    uint8_t* hVF = nullptr;
+    bool tablesDirty = true;
+    // graphs
+    hipGraphExec_t gMulti = nullptr, gOne = nullptr;
+    int gSteps = 0, gIntegrator = -1, gResid = -1;
+    // dispatch
+    StageFn stageFn = nullptr;
+    PerStepFn sourceFn = nullptr, postFn = nullptr;
+    int64_t deviceBytes = 0;
+    double lastRunMs = 0.0;
+    // per-run settings read by the launchers
+    int curResid = 1;
+    double curDt = 0.0, gDt = -1.0;
+};
+
+namespace {
+
+#define HIPCHK(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            throw std::runtime_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+        }                                                                               \
+    } while (0)
+
+template <typename T>
+T* devAlloc(dg_context* c, size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc(&p, n * sizeof(T)));
+    c->deviceBytes += static_cast<int64_t>(n * sizeof(T));
+    return static_cast<T*>(p);
+}
+
+void devFree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+template <typename T>
+dgk::Physics<T> makePhysics(const dg_physics& ph) {
+    dgk::Physics<T> o;
+    for (int k = 0; k < 3; ++k) o.v0[k] = static_cast<T>(ph.v0[k]);
+    o.rho0 = static_cast<T>(ph.rho0);
+    o.c0 = static_cast<T>(ph.c0);
+    o.rc2 = static_cast<T>(ph.rho0 * ph.c0 * ph.c0);
+    o.irho0 = static_cast<T>(1.0 / ph.rho0);
+    o.irc0 = static_cast<T>(1.0 / (ph.rho0 * ph.c0));
+    o.ic2 = static_cast<T>(1.0 / (ph.c0 * ph.c0));
+    return o;
+}
+
+template <int DIM, int P, int NF, typename T, bool COMP>
+void launchStage(dg_context* c, int stage, const void* qin, void* qout, void* uout, bool snap) {
+    dgk::StageArgs<T> a{};
+    a.qin = static_cast<const T*>(qin);
+    a.qout = static_cast<T*>(qout);
+    a.u = static_cast<T*>(c->u);
+    a.uout = static_cast<T*>(uout);
+    a.ulo = COMP ? static_cast<T*>(c->ulo) : nullptr;
+    a.acc = static_cast<T*>(c->acc);
+    a.geo = static_cast<const T*>(c->geo);
+    a.nbr = c->nbr;
+    a.code = c->code;
+    a.ops = static_cast<const T*>(c->ops);
+    a.iops = c->iops;
+    a.snap = snap ? static_cast<const T*>(c->snap) : nullptr;
+    a.snaplo = (snap && COMP) ? static_cast<const T*>(c->snaplo) : nullptr;
+    a.resPartial = c->resPartial;
+    a.K = c->K;
+    a.dt = static_cast<T>(c->curDt);
+    a.stage = stage;
+    a.ph = makePhysics<T>(c->phys);
+    hipLaunchKernelGGL((dgk::stageKernel<DIM, P, NF, T, COMP>), dim3(static_cast<unsigned>(c->nblocks)),
+                       dim3(dgk::kBlock), 0, c->stream, a);
+}
+
+template <typename T, bool COMP>
+void launchSource(dg_context* c) {
+    if (c->srcNodes.empty()) return;
+    dgk::SourceArgs s{};
+    s.nodeOffsets = c->dSrcOff;
+    s.nodeIdx = c->dSrcIdx;
+    s.params = c->dSrcParams;
+    s.tTable = c->dT;
+    s.vTable = c->dV;
+    s.vFlag = c->dVF;
+    s.stepCounter = c->counter;
+    s.nsrc = static_cast<int>(c->srcNodes.size());
+    s.nf = c->nf;
+    hipLaunchKernelGGL((dgk::sourceKernel<T, COMP>), dim3(1, static_cast<unsigned>(std::min<size_t>(c->srcNodes.size(), 64))),
+                       dim3(64), 0, c->stream, s, static_cast<T*>(c->u), static_cast<T*>(COMP ? c->ulo : nullptr),
+                       1.0 / (c->phys.c0 * c->phys.c0));
+}
+
+template <typename T, bool COMP>
+void launchPost(dg_context* c) {
+    dgk::PostArgs p{};
+    p.resPartial = c->resPartial;
+    p.nblocks = c->nblocks;
+    p.invN = 1.0 / static_cast<double>(c->K * c->Np);
+    p.obsOffsets = c->dObsOff;
+    p.obsIdx = c->dObsIdx;
+    p.obsW = c->dObsW;
+    p.nobs = static_cast<int>(c->obsNodes.size());
+    p.nf = c->nf;
+    p.resRing = c->resRing;
+    p.obsRing = c->obsRing;
+    p.nonfinite = c->nonfinite;
+    p.stepCounter = c->counter;
+    p.computeResiduals = c->curResid;
+    hipLaunchKernelGGL((dgk::postKernel<T, COMP>), dim3(1), dim3(256), 0, c->stream, p,
+                       static_cast<const T*>(c->u), static_cast<const T*>(COMP ? c->ulo : nullptr));
+}
+
+template <int DIM, int P, int NF>
+bool pick(dg_context* c) {
+    if (c->precision == DG_FP64) {
+        c->stageFn = &launchStage<DIM, P, NF, double, false>;
+        c->sourceFn = &launchSource<double, false>;
+        c->postFn = &launchPost<double, false>;
+    } else {
+        c->stageFn = &launchStage<DIM, P, NF, float, true>;
+        c->sourceFn = &launchSource<float, true>;
+        c->postFn = &launchPost<float, true>;
+    }
+    return true;
+}
+
+template <int DIM, int NF>
+bool pickP(dg_context* c) {
+    switch (c->p) {
+        case 1: return pick<DIM, 1, NF>(c);
+        case 2: return pick<DIM, 2, NF>(c);
+        case 3: return pick<DIM, 3, NF>(c);
+        case 4: return pick<DIM, 4, NF>(c);
+        case 5: return pick<DIM, 5, NF>(c);
+        default: return false;
+    }
+}
+
+bool pickKernels(dg_context* c) {
+    if (c->dim == 2) return c->nf == 4 ? pickP<2, 4>(c) : pickP<2, 5>(c);
+    return c->nf == 4 ? pickP<3, 4>(c) : pickP<3, 5>(c);
+}
+
+template <int DIM, int P>
+int opTotal() {
+    return dgk::OpLayout<DIM, P>::kTotal;
+}
+
+// Operator tables in the kernel's layout (see dg_kernels.hpp OpLayout).
+void buildOps(const dg::RefElement& R, std::vector<double>& ops, std::vector<int32_t>& iops) {
+    const int Np = R.Np, dim = R.dim, NFT = R.Nfaces * R.Nfp;
+    ops.assign(static_cast<size_t>(Np) * Np * dim + static_cast<size_t>(NFT) * Np + R.Nfaces * 3, 0.0);
+    size_t o = 0;
+    for (int j = 0; j < Np; ++j)
+        for (int i = 0; i < Np; ++i)
+            for (int r = 0; r < dim; ++r) ops[o++] = R.Dw[(static_cast<size_t>(r) * Np + i) * Np + j];
+    for (int fa = 0; fa < NFT; ++fa)
+        for (int i = 0; i < Np; ++i) ops[o++] = R.LIFT[static_cast<size_t>(i) * NFT + fa];
+    for (int f = 0; f < R.Nfaces; ++f)
+        for (int k = 0; k < 3; ++k) ops[o++] = R.Nref[f * 3 + k];
+    iops.clear();
+    for (int v : R.fmask) iops.push_back(v);
+    for (int v : R.facePerm) iops.push_back(v);
+}
+
+template <typename T>
+void uploadAs(dg_context* c, void* dst, const std::vector<double>& src) {
+    std::vector<T> tmp(src.size());
+    for (size_t i = 0; i < src.size(); ++i) tmp[i] = static_cast<T>(src[i]);
+    HIPCHK(hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+void releaseGraphs(dg_context* c) {
+    if (c->gMulti) (void)hipGraphExecDestroy(c->gMulti);
+    if (c->gOne) (void)hipGraphExecDestroy(c->gOne);
+    c->gMulti = c->gOne = nullptr;
+    c->gSteps = 0;
+    c->gIntegrator = -1;
+}
+
+void freeTables(dg_context* c) {
+    devFree(c->dSrcOff);
+    devFree(c->dSrcIdx);
+    devFree(c->dSrcParams);
+    devFree(c->dObsOff);
+    devFree(c->dObsIdx);
+    devFree(c->dObsW);
+    c->dSrcOff = c->dSrcIdx = c->dObsOff = c->dObsIdx = nullptr;
+    c->dSrcParams = c->dObsW = nullptr;
+}
+
+template <typename V>
+V* uploadVec(dg_context* c, const std::vector<V>& v) {
+    V* d = devAlloc<V>(c, v.size());
+    if (!v.empty()) HIPCHK(hipMemcpy(d, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice));
+    return d;
+}
+
+void syncTables(dg_context* c) {
+    if (!c->tablesDirty) return;
+    releaseGraphs(c);
+    freeTables(c);
+    std::vector<int64_t> off{0}, idx;
+    std::vector<double> prm;
+    for (size_t s = 0; s < c->srcNodes.size(); ++s) {
+        idx.insert(idx.end(), c->srcNodes[s].begin(), c->srcNodes[s].end());
+        off.push_back(static_cast<int64_t>(idx.size()));
+        prm.insert(prm.end(), c->srcParams[s].begin(), c->srcParams[s].end());
+    }
+    c->dSrcOff = uploadVec(c, off);
+    c->dSrcIdx = uploadVec(c, idx);
+    c->dSrcParams = uploadVec(c, prm);
+    std::vector<int64_t> ooff{0}, oidx;
+    std::vector<double> ow;
+    for (size_t o = 0; o < c->obsNodes.size(); ++o) {
+        oidx.insert(oidx.end(), c->obsNodes[o].begin(), c->obsNodes[o].end());
+        ow.insert(ow.end(), c->obsW[o].begin(), c->obsW[o].end());
+        ooff.push_back(static_cast<int64_t>(oidx.size()));
+    }
+    c->dObsOff = uploadVec(c, ooff);
+    c->dObsIdx = uploadVec(c, oidx);
+    c->dObsW = uploadVec(c, ow);
+    // rings and per-step tables
+    const int64_t ns = std::max<int64_t>(1, static_cast<int64_t>(c->srcNodes.size()));
+    const int64_t no = std::max<int64_t>(1, static_cast<int64_t>(c->obsNodes.size()));
+    devFree(c->dT);
+    devFree(c->dV);
+    devFree(c->dVF);
+    devFree(c->obsRing);
+    if (c->hV) (void)hipHostFree(c->hV);
+    if (c->hVF) (void)hipHostFree(c->hVF);
+    if (c->hObs) (void)hipHostFree(c->hObs);
+    c->dT = devAlloc<double>(c, c->ringSlots);
+    c->dV = devAlloc<double>(c, c->ringSlots * ns);
+    c->dVF = devAlloc<uint8_t>(c, c->ringSlots * ns);
+    c->obsRing = devAlloc<double>(c, c->ringSlots * no * 4);
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hV), sizeof(double) * c->ringSlots * ns));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hVF), c->ringSlots * ns));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hObs), sizeof(double) * c->ringSlots * no * 4));
+    std::memset(c->hV, 0, sizeof(double) * c->ringSlots * ns);
+    std::memset(c->hVF, 0, c->ringSlots * ns);
+    c->tablesDirty = false;
+}
+
+// Launch the kernels of one step on the context stream.
+void launchStep(dg_context* c, int integrator, bool resid) {
+    c->sourceFn(c);
+    if (integrator == DG_RK4) {
+        c->stageFn(c, dgk::kStageRK1, c->u, c->qa, c->u, false);
+        c->stageFn(c, dgk::kStageRK2, c->qa, c->qb, c->u, false);
+        c->stageFn(c, dgk::kStageRK3, c->qb, c->qa, c->u, false);
+        c->stageFn(c, dgk::kStageRK4, c->qa, nullptr, c->u, resid);
+    } else {
+        c->stageFn(c, dgk::kStageEuler, c->u, nullptr, c->qb, resid);
+        HIPCHK(hipMemcpyAsync(c->u, c->qb, static_cast<size_t>(c->K) * c->Np * c->nf * c->tsize,
+                              hipMemcpyDeviceToDevice, c->stream));
+    }
+    c->postFn(c);
+}
+
+void captureGraphs(dg_context* c, int integrator, bool resid, int steps) {
+    releaseGraphs(c);
+    auto capture = [&](int n) {
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int s = 0; s < n; ++s) launchStep(c, integrator, resid);
+        HIPCHK(hipStreamEndCapture(c->stream, &g));
+        hipGraphExec_t ex;
+        HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        return ex;
+    };
+    c->gOne = capture(1);
+    c->gMulti = steps > 1 ? capture(steps) : nullptr;
+    c->gSteps = steps;
+    c->gIntegrator = integrator;
+    c->gResid = resid ? 1 : 0;
+}
+
+// Caller layout [nf][K*Np] (file node order) <-> device layout [K][Np canonical][nf].
+void toDeviceLayout(const dg_context* c, const double* u, std::vector<double>& out) {
+    const int64_t N = c->K * c->Np;
+    out.assign(static_cast<size_t>(N) * c->nf, 0.0);
+    for (int64_t e = 0; e < c->K; ++e)
+        for (int a = 0; a < c->Np; ++a) {
+            const int64_t cn = e * c->Np + c->mesh.toCanon[e * c->Np + a];
+            for (int f = 0; f < c->nf; ++f) out[cn * c->nf + f] = u[f * N + e * c->Np + a];
+        }
+}
+
+void fromDeviceLayout(const dg_context* c, const std::vector<double>& dev, double* u) {
+    const int64_t N = c->K * c->Np;
+    for (int64_t e = 0; e < c->K; ++e)
+        for (int a = 0; a < c->Np; ++a) {
+            const int64_t cn = e * c->Np + c->mesh.toCanon[e * c->Np + a];
+            for (int f = 0; f < c->nf; ++f) u[f * N + e * c->Np + a] = dev[cn * c->nf + f];
+        }
+}
+
+void uploadState(dg_context* c, const std::vector<double>& dev, void* hiBuf, void* loBuf) {
+    if (c->precision == DG_FP64) {
+        HIPCHK(hipMemcpy(hiBuf, dev.data(), dev.size() * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> hi(dev.size()), lo(dev.size());
+        for (size_t i = 0; i < dev.size(); ++i) {
+            hi[i] = static_cast<float>(dev[i]);
+            lo[i] = static_cast<float>(dev[i] - static_cast<double>(hi[i]));
+        }
+        HIPCHK(hipMemcpy(hiBuf, hi.data(), hi.size() * 4, hipMemcpyHostToDevice));
+        if (loBuf) HIPCHK(hipMemcpy(loBuf, lo.data(), lo.size() * 4, hipMemcpyHostToDevice));
+    }
+}
+
+void downloadState(dg_context* c, const void* hiBuf, const void* loBuf, std::vector<double>& dev) {
+    const size_t n = static_cast<size_t>(c->K) * c->Np * c->nf;
+    dev.resize(n);
+    if (c->precision == DG_FP64) {
+        HIPCHK(hipMemcpy(dev.data(), hiBuf, n * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> hi(n), lo(n, 0.0f);
+        HIPCHK(hipMemcpy(hi.data(), hiBuf, n * 4, hipMemcpyDeviceToHost));
+        if (loBuf) HIPCHK(hipMemcpy(lo.data(), loBuf, n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) dev[i] = static_cast<double>(hi[i]) + static_cast<double>(lo[i]);
+    }
+}
+
+dg_status fail(dg_context* c, dg_status code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+void destroyContext(dg_context* c) {
+    if (!c) return;
+    if (c->device >= 0) (void)hipSetDevice(c->device);
+    releaseGraphs(c);
+    freeTables(c);
+    for (void* p : {c->u, c->ulo, c->qa, c->qb, c->acc, c->snap, c->snaplo, c->geo, c->ops})
+        devFree(p);
+    devFree(c->nbr);
+    devFree(c->iops);
+    devFree(c->code);
+    devFree(c->resPartial);
+    devFree(c->dT);
+    devFree(c->dV);
+    devFree(c->dVF);
+    devFree(c->resRing);
+    devFree(c->obsRing);
+    devFree(c->nonfinite);
+    devFree(c->counter);
+    for (void* h : {static_cast<void*>(c->hT), static_cast<void*>(c->hV), static_cast<void*>(c->hRes),
+                    static_cast<void*>(c->hObs), static_cast<void*>(c->hVF)})
+        if (h) (void)hipHostFree(h);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+}  // namespace
+
+extern "C" {
+
+dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_options* opt, dg_context** out) {
+    if (!out) return DG_ERR_INVALID;
+    *out = nullptr;
+    if (!m || !ph || !m->node_coords || !m->element_vertices) {
+        g_createError = "dg_create: null argument";
+        return DG_ERR_INVALID;
+    }
+    if (m->dim != 2 && m->dim != 3) {
+        g_createError = "dg_create: dim must be 2 or 3";
+        return DG_ERR_UNSUPPORTED;
+    }
+    if (m->order < 1 || m->order > 5) {
+        g_createError = "dg_create: order must be in 1..5";
+        return DG_ERR_UNSUPPORTED;
+    }
+    if (m->nodes_per_element != dg::numNodes(m->dim, m->order)) {
+        g_createError = "dg_create: nodes_per_element does not match the order";
+        return DG_ERR_INVALID;
+    }
+    if (!(ph->c0 > 0) || !(ph->rho0 > 0)) {
+        g_createError = "dg_create: c0 and rho0 must be positive";
+        return DG_ERR_INVALID;
+    }
+    dg_context* c = new dg_context();
+    c->device = -1;
+    try {
+        c->opt = opt ? *opt : dg_options{DG_FP64, 1, 16, 0};
+        if (c->opt.steps_per_graph < 1) c->opt.steps_per_graph = 1;
+        c->precision = c->opt.precision == DG_MIXED ? DG_MIXED : DG_FP64;
+        c->tsize = c->precision == DG_FP64 ? 8 : 4;
+        c->phys = *ph;
+        c->nf = ph->fields == DG_FIELDS_LEE5 ? 5 : 4;
+        c->dim = m->dim;
+        c->p = m->order;
+        c->K = m->num_elements;
+        c->mesh = dg::buildDGMesh(m->dim, m->order, m->num_elements, m->node_coords, m->element_vertices, m->face_bc);
+        const dg::RefElement& R = dg::refElement(m->dim, m->order);
+        c->Np = R.Np;
+        c->Nfp = R.Nfp;
+        c->Nfc = R.Nfaces;
+        if (c->K > INT32_MAX / 2) throw std::runtime_error("too many elements for 32-bit neighbour ids");
+        if (!pickKernels(c)) {
+            delete c;
+            g_createError = "dg_create: no kernel built for this (dim, order)";
+            return DG_ERR_UNSUPPORTED;
+        }
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+            destroyContext(c);
+            g_createError = "dg_create: no HIP device available";
+            return DG_ERR_DEVICE;
+        }
+        c->device = c->opt.device;
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        const int EB = dgk::kBlock / c->Np;
+        c->nblocks = (c->K + EB - 1) / EB;
+        const size_t nstate = static_cast<size_t>(c->K) * c->Np * c->nf;
+        auto alloc = [&](size_t n) -> void* {
+            return c->precision == DG_FP64 ? static_cast<void*>(devAlloc<double>(c, n))
+                                           : static_cast<void*>(devAlloc<float>(c, n));
+        };
+        c->u = alloc(nstate);
+        c->qa = alloc(nstate);
+        c->qb = alloc(nstate);
+        c->acc = alloc(nstate);
+        c->snap = alloc(nstate);
+        if (c->precision == DG_MIXED) {
+            c->ulo = alloc(nstate);
+            c->snaplo = alloc(nstate);
+        }
+        HIPCHK(hipMemset(c->u, 0, nstate * c->tsize));
+        if (c->ulo) HIPCHK(hipMemset(c->ulo, 0, nstate * c->tsize));
+        // geometry: [K][dim*dim] dxi_r/dx_c
+        std::vector<double> geo(static_cast<size_t>(c->K) * c->dim * c->dim);
+        for (int64_t e = 0; e < c->K; ++e)
+            for (int r = 0; r < c->dim; ++r)
+                for (int q = 0; q < c->dim; ++q) geo[(e * c->dim + r) * c->dim + q] = c->mesh.Jinv[e * 9 + r * 3 + q];
+        c->geo = alloc(geo.size());
+        std::vector<double> ops;
+        std::vector<int32_t> iops;
+        buildOps(R, ops, iops);
+        c->ops = alloc(ops.size());
+        if (c->precision == DG_FP64) {
+            uploadAs<double>(c, c->geo, geo);
+            uploadAs<double>(c, c->ops, ops);
+        } else {
+            uploadAs<float>(c, c->geo, geo);
+            uploadAs<float>(c, c->ops, ops);
+        }
+        c->iops = uploadVec(c, iops);
+        c->nbr = uploadVec(c, c->mesh.nbr);
+        c->code = uploadVec(c, c->mesh.nbrCode);
+        c->resPartial = devAlloc<double>(c, static_cast<size_t>(c->nblocks) * 5);
+        HIPCHK(hipMemset(c->resPartial, 0, static_cast<size_t>(c->nblocks) * 5 * sizeof(double)));
+        c->ringSlots = 4096;
+        c->resRing = devAlloc<double>(c, c->ringSlots * 5);
+        c->nonfinite = devAlloc<int>(c, 1);
+        c->counter = devAlloc<int64_t>(c, 1);
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hT), sizeof(double) * c->ringSlots));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hRes), sizeof(double) * c->ringSlots * 5));
+        c->tablesDirty = true;
+        syncTables(c);
+        HIPCHK(hipDeviceSynchronize());
+    } catch (const std::exception& ex) {
+        g_createError = std::string("dg_create: ") + ex.what();
+        bool meshErr = c->stream == nullptr;
+        destroyContext(c);
+        return meshErr ? DG_ERR_MESH : DG_ERR_DEVICE;
+    }
+    *out = c;
+    return DG_OK;
+}
+
+void dg_destroy(dg_context* c) { destroyContext(c); }
+
+const char* dg_last_error(const dg_context* c) { return c ? c->err.c_str() : g_createError.c_str(); }
+
+dg_status dg_set_state(dg_context* c, const double* u) {
+    if (!c || !u) return DG_ERR_INVALID;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        std::vector<double> dev;
+        toDeviceLayout(c, u, dev);
+        uploadState(c, dev, c->u, c->ulo);
+    } catch (const std::exception& ex) {
+        return fail(c, DG_ERR_DEVICE, ex.what());
+    }
+    return DG_OK;
+}
+
+dg_status dg_get_state(dg_context* c, double* u) {
+    if (!c || !u) return DG_ERR_INVALID;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<double> dev;
+        downloadState(c, c->u, c->ulo, dev);
+        fromDeviceLayout(c, dev, u);
+    } catch (const std::exception& ex) {
+        return fail(c, DG_ERR_DEVICE, ex.what());
+    }
+    return DG_OK;
+}
+
+dg_status dg_add_gaussian(dg_context* c, double x, double y, double z, double size, double amp) {
+    if (!c || !(size > 0)) return DG_ERR_INVALID;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<double> dev;
+        downloadState(c, c->u, c->ulo, dev);
+        const int64_t N = c->K * c->Np;
+        for (int64_t n = 0; n < N; ++n) {
+            const double* X = &c->mesh.nodes[n * 3];
+            double v = amp * std::exp(-((X[0] - x) * (X[0] - x) + (X[1] - y) * (X[1] - y) + (X[2] - z) * (X[2] - z)) /
+                                      size);
+            dev[n * c->nf] += v;
+            if (c->nf == 5) dev[n * c->nf + 4] += v / (c->phys.c0 * c->phys.c0);
+        }
+        uploadState(c, dev, c->u, c->ulo);
+    } catch (const std::exception& ex) {
+        return fail(c, DG_ERR_DEVICE, ex.what());
+    }
+    return DG_OK;
+}
+
+static dg_status addSource(dg_context* c, double x, double y, double z, double size, const std::array<double, 5>& prm,
+                           TableSource ts, int64_t* num) {
+    if (!c) return DG_ERR_INVALID;
+    std::vector<int64_t> nodes;
+    const int64_t N = c->K * c->Np;
+    for (int64_t n = 0; n < N; ++n) {
+        const double* X = &c->mesh.nodes[n * 3];
+        if (std::pow(X[0] - x, 2) + std::pow(X[1] - y, 2) + std::pow(X[2] - z, 2) < std::pow(size, 2))
+            nodes.push_back(n);
+    }
+    c->srcNodes.push_back(nodes);
+    c->srcParams.push_back(prm);
+    c->srcTables.push_back(ts);
+    c->tablesDirty = true;
+    if (num) *num = static_cast<int64_t>(nodes.size());
+    return DG_OK;
+}
+
+dg_status dg_add_source_monopole(dg_context* c, double x, double y, double z, double size, double amp, double freq,
+                                 double phase, double duration, int64_t* num) {
+    return addSource(c, x, y, z, size, {0.0, amp, freq, phase, duration}, TableSource{nullptr, nullptr}, num);
+}
+
+dg_status dg_add_source_table(dg_context* c, double x, double y, double z, double size, dg_value_fn fn, void* user,
+                              int64_t* num) {
+    if (!fn) return fail(c, DG_ERR_INVALID, "dg_add_source_table: null value function");
+    return addSource(c, x, y, z, size, {1.0, 0, 0, 0, 0}, TableSource{fn, user}, num);
+}
+
+dg_status dg_add_observer(dg_context* c, double x, double y, double z, double size, int64_t* num) {
+    if (!c) return DG_ERR_INVALID;
+    std::vector<int64_t> nodes;
+    std::vector<double> w;
+    const int64_t N = c->K * c->Np;
+    for (int64_t n = 0; n < N; ++n) {
+        const double* X = &c->mesh.nodes[n * 3];
+        double d = std::sqrt(std::pow(X[0] - x, 2) + std::pow(X[1] - y, 2) + std::pow(X[2] - z, 2));
+        if (d < size) {
+            nodes.push_back(n);
+            w.push_back(1.0 / (std::pow(d, 2) + 1.0e-12));
+        }
+    }
+    c->obsNodes.push_back(nodes);
+    c->obsW.push_back(w);
+    c->tablesDirty = true;
+    if (num) *num = static_cast<int64_t>(nodes.size());
+    return DG_OK;
+}
+
+dg_status dg_run(dg_context* c, const dg_run_params* prm, dg_step_fn on_step, dg_save_fn on_save, void* user,
+                 int64_t* steps_done) {
+    if (!c || !prm) return DG_ERR_INVALID;
+    if (steps_done) *steps_done = 0;
+    if (!(prm->dt > 0)) return fail(c, DG_ERR_INVALID, "dg_run: dt must be positive");
+    if (prm->integrator != DG_RK4 && prm->integrator != DG_EULER)
+        return fail(c, DG_ERR_INVALID, "dg_run: unknown integrator");
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        syncTables(c);
+        // The reference loop (src/solver.cpp:413-414): double accumulation of t and tDisplay.
+        std::vector<double> times;
+        std::vector<uint8_t> isSave;
+        {
+            double t = prm->t_start, step = 0, tDisplay = 0;
+            for (; t <= prm->t_end; t += prm->dt, tDisplay += prm->dt, ++step) {
+                if (prm->max_steps >= 0 && static_cast<int64_t>(times.size()) >= prm->max_steps) break;
+                bool save = false;
+                if (tDisplay >= prm->t_rate || step == 0) {
+                    tDisplay = 0;
+                    save = true;
+                }
+                times.push_back(t);
+                isSave.push_back(save);
+            }
+        }
+        const bool resid = prm->compute_residuals != 0;
+        c->curResid = resid ? 1 : 0;
+        c->curDt = prm->dt;
+        const bool useGraph = c->opt.use_graph != 0;
+        if (useGraph && (c->gOne == nullptr || c->gIntegrator != prm->integrator || c->gResid != (resid ? 1 : 0) ||
+                         c->gSteps != c->opt.steps_per_graph || c->gDt != prm->dt))
+            captureGraphs(c, prm->integrator, resid, c->opt.steps_per_graph), c->gDt = prm->dt;
+        const int64_t nsteps = static_cast<int64_t>(times.size());
+        const int ns = static_cast<int>(c->srcNodes.size());
+        const int no = static_cast<int>(c->obsNodes.size());
+        const size_t stateN = static_cast<size_t>(c->K) * c->Np * c->nf;
+        hipEvent_t ev0, ev1;
+        HIPCHK(hipEventCreate(&ev0));
+        HIPCHK(hipEventCreate(&ev1));
+        double devMs = 0.0;
+        std::vector<double> stateHost;
+        int64_t done = 0;
+        while (done < nsteps) {
+            int64_t end = done + 1;
+            while (end < nsteps && !isSave[end] && end - done < c->ringSlots) ++end;
+            const int64_t n = end - done;
+            if (isSave[done]) {
+                HIPCHK(hipMemcpyAsync(c->snap, c->u, stateN * c->tsize, hipMemcpyDeviceToDevice, c->stream));
+                if (c->snaplo)
+                    HIPCHK(hipMemcpyAsync(c->snaplo, c->ulo, stateN * c->tsize, hipMemcpyDeviceToDevice, c->stream));
+                if (on_save) {
+                    HIPCHK(hipStreamSynchronize(c->stream));
+                    std::vector<double> dev;
+                    downloadState(c, c->u, c->ulo, dev);
+                    stateHost.resize(stateN);
+                    fromDeviceLayout(c, dev, stateHost.data());
+                    if (on_save(user, done, times[done], stateHost.data()) != 0) {
+                        HIPCHK(hipEventDestroy(ev0));
+                        HIPCHK(hipEventDestroy(ev1));
+                        return fail(c, DG_ERR_CALLBACK, "dg_run: save callback failed");
+                    }
+                }
+            }
+            for (int64_t s = 0; s < n; ++s) {
+                c->hT[s] = times[done + s];
+                for (int q = 0; q < ns; ++q) {
+                    double v = 0.0;
+                    int on = 0;
+                    if (c->srcTables[q].fn) on = c->srcTables[q].fn(c->srcTables[q].user, times[done + s], &v) != 0;
+                    c->hV[s * ns + q] = v;
+                    c->hVF[s * ns + q] = static_cast<uint8_t>(on);
+                }
+            }
+            HIPCHK(hipMemcpyAsync(c->dT, c->hT, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+            if (ns > 0) {
+                HIPCHK(hipMemcpyAsync(c->dV, c->hV, sizeof(double) * n * ns, hipMemcpyHostToDevice, c->stream));
+                HIPCHK(hipMemcpyAsync(c->dVF, c->hVF, n * ns, hipMemcpyHostToDevice, c->stream));
+            }
+            hipLaunchKernelGGL(dgk::setCounterKernel, dim3(1), dim3(1), 0, c->stream, c->counter, int64_t(0),
+                               c->nonfinite);
+            HIPCHK(hipEventRecord(ev0, c->stream));
+            if (useGraph) {
+                int64_t s = 0;
+                if (c->gMulti)
+                    for (; s + c->gSteps <= n; s += c->gSteps) HIPCHK(hipGraphLaunch(c->gMulti, c->stream));
+                for (; s < n; ++s) HIPCHK(hipGraphLaunch(c->gOne, c->stream));
+            } else {
+                for (int64_t s = 0; s < n; ++s) launchStep(c, prm->integrator, resid);
+            }
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(ev1, c->stream));
+            HIPCHK(hipMemcpyAsync(c->hRes, c->resRing, sizeof(double) * n * 5, hipMemcpyDeviceToHost, c->stream));
+            if (no > 0)
+                HIPCHK(hipMemcpyAsync(c->hObs, c->obsRing, sizeof(double) * n * no * 4, hipMemcpyDeviceToHost,
+                                      c->stream));
+            int flag = -1;
+            HIPCHK(hipMemcpyAsync(&flag, c->nonfinite, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+            devMs += ms;
+            int64_t lim = n;
+            if (flag >= 0 && resid) lim = flag + 1;
+            for (int64_t s = 0; s < lim; ++s) {
+                if (on_step) {
+                    dg_step_record rec{done + s, times[done + s], &c->hRes[s * 5], no, no ? &c->hObs[s * no * 4] : nullptr};
+                    if (on_step(user, &rec) != 0) {
+                        (void)hipEventDestroy(ev0);
+                        (void)hipEventDestroy(ev1);
+                        if (steps_done) *steps_done = done + s + 1;
+                        return fail(c, DG_ERR_CALLBACK, "dg_run: step callback failed");
+                    }
+                }
+            }
+            if (flag >= 0 && resid) {
+                (void)hipEventDestroy(ev0);
+                (void)hipEventDestroy(ev1);
+                if (steps_done) *steps_done = done + flag + 1;
+                c->lastRunMs = devMs;
+                return fail(c, DG_ERR_NONFINITE,
+                            "dg_run: non-finite residual at step " + std::to_string(done + flag));
+            }
+            done = end;
+        }
+        (void)hipEventDestroy(ev0);
+        (void)hipEventDestroy(ev1);
+        c->lastRunMs = devMs;
+        if (steps_done) *steps_done = done;
+    } catch (const std::exception& ex) {
+        return fail(c, DG_ERR_DEVICE, ex.what());
+    }
+    return DG_OK;
+}
+
+struct CollectCtx {
+    double *times, *res, *obs;
+    int64_t cap;
+};
+
+static int collectStep(void* user, const dg_step_record* r) {
+    auto* cc = static_cast<CollectCtx*>(user);
+    if (r->step >= cc->cap) return 1;
+    if (cc->times) cc->times[r->step] = r->t;
+    if (cc->res) std::memcpy(cc->res + r->step * 5, r->residuals, 5 * sizeof(double));
+    if (cc->obs && r->num_observers > 0)
+        std::memcpy(cc->obs + r->step * r->num_observers * 4, r->observers, sizeof(double) * 4 * r->num_observers);
+    return 0;
+}
+
+dg_status dg_run_collect(dg_context* c, const dg_run_params* prm, double* times, double* residuals, double* observers,
+                         int64_t capacity, int64_t* steps_done) {
+    if (!c || !prm) return DG_ERR_INVALID;
+    dg_run_params p2 = *prm;
+    if (p2.max_steps < 0 || p2.max_steps > capacity) p2.max_steps = capacity;
+    CollectCtx cc{times, residuals, observers, capacity};
+    return dg_run(c, &p2, collectStep, nullptr, &cc, steps_done);
+}
+
+dg_status dg_rhs(dg_context* c, const double* u, double* k, double dt) {
+    if (!c || !u || !k) return DG_ERR_INVALID;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        std::vector<double> dev;
+        toDeviceLayout(c, u, dev);
+        uploadState(c, dev, c->qa, nullptr);
+        c->curDt = dt;
+        c->stageFn(c, dgk::kStageRHS, c->qa, c->qb, c->u, false);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<double> out;
+        downloadState(c, c->qb, nullptr, out);
+        fromDeviceLayout(c, out, k);
+    } catch (const std::exception& ex) {
+        return fail(c, DG_ERR_DEVICE, ex.what());
+    }
+    return DG_OK;
+}
+
+dg_status dg_get_info(const dg_context* c, dg_info* info) {
+    if (!c || !info) return DG_ERR_INVALID;
+    info->dim = c->dim;
+    info->order = c->p;
+    info->nodes_per_element = c->Np;
+    info->nodes_per_face = c->Nfp;
+    info->fields = c->nf;
+    info->precision = c->precision;
+    info->num_elements = c->K;
+    info->num_faces = c->mesh.numFaces;
+    info->num_boundary_faces = c->mesh.numBoundaryFaces;
+    double r = 1e300;
+    for (double v : c->mesh.inradius) r = std::min(r, v);
+    info->min_inradius = r;
+    info->device_bytes = c->deviceBytes;
+    return DG_OK;
+}
+
+dg_status dg_profile_steps(dg_context* c, double dt, int32_t steps, double* stage_ms, double* source_ms,
+                           double* post_ms) {
+    if (!c || steps < 1) return DG_ERR_INVALID;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        syncTables(c);
+        c->curDt = dt;
+        c->curResid = 1;
+        std::vector<double> tt(std::min<int64_t>(steps, c->ringSlots), 0.0);
+        for (size_t s = 0; s < tt.size(); ++s) c->hT[s] = s * dt;
+        HIPCHK(hipMemcpyAsync(c->dT, c->hT, sizeof(double) * tt.size(), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(dgk::setCounterKernel, dim3(1), dim3(1), 0, c->stream, c->counter, int64_t(0), c->nonfinite);
+        std::vector<hipEvent_t> ev(14);
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        int n = static_cast<int>(tt.size());
+        for (int s = 0; s < n; ++s) {
+            HIPCHK(hipEventRecord(ev[0], c->stream));
+            c->sourceFn(c);
+            HIPCHK(hipEventRecord(ev[1], c->stream));
+            c->stageFn(c, dgk::kStageRK1, c->u, c->qa, c->u, false);
+            HIPCHK(hipEventRecord(ev[2], c->stream));
+            c->stageFn(c, dgk::kStageRK2, c->qa, c->qb, c->u, false);
+            HIPCHK(hipEventRecord(ev[3], c->stream));
+            c->stageFn(c, dgk::kStageRK3, c->qb, c->qa, c->u, false);
+            HIPCHK(hipEventRecord(ev[4], c->stream));
+            c->stageFn(c, dgk::kStageRK4, c->qa, nullptr, c->u, true);
+            HIPCHK(hipEventRecord(ev[5], c->stream));
+            c->postFn(c);
+            HIPCHK(hipEventRecord(ev[6], c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            float ms;
+            HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            acc[4] += ms;
+            for (int k = 0; k < 4; ++k) {
+                HIPCHK(hipEventElapsedTime(&ms, ev[1 + k], ev[2 + k]));
+                acc[k] += ms;
+            }
+            HIPCHK(hipEventElapsedTime(&ms, ev[5], ev[6]));
+            acc[5] += ms;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        HIPCHK(hipGetLastError());
+        if (stage_ms)
+            for (int k = 0; k < 4; ++k) stage_ms[k] = acc[k] / n;
+        if (source_ms) *source_ms = acc[4] / n;
+        if (post_ms) *post_ms = acc[5] / n;
+    } catch (const std::exception& ex) {
+        return fail(c, DG_ERR_DEVICE, ex.what());
+    }
+    return DG_OK;
+}
+
+double dg_last_run_device_ms(const dg_context* c) { return c ? c->lastRunMs : 0.0; }
+
+}  // extern "C"

@@ -1,0 +1,85 @@
+// Host-only entry points of the C ABI (mesh generation and the BC rule).
+#include <cstring>
+#include <stdexcept>
+
+#include "../../../include/dgfem_amd.h"
+#include "mesh.hpp"
+
+extern "C" {
+
+dg_status dg_box_mesh_size(int32_t dim, int32_t n, int32_t order, int64_t* num_elements, int32_t* nodes_per_element) {
+    if ((dim != 2 && dim != 3) || n < 1 || order < 1 || order > 8) return DG_ERR_INVALID;
+    if (num_elements) *num_elements = dim == 2 ? int64_t(2) * n * n : int64_t(6) * n * n * n;
+    if (nodes_per_element) *nodes_per_element = dg::numNodes(dim, order);
+    return DG_OK;
+}
+
+dg_status dg_box_mesh_fill(int32_t dim, int32_t n, int32_t order, double jitter, uint64_t seed, double lo, double hi,
+                           double* node_coords, int64_t* element_vertices, int8_t* face_side) {
+    try {
+        dg::SimplexMesh m = dg::generateBoxMesh(dim, n, order, jitter, seed, lo, hi);
+        if (node_coords) std::memcpy(node_coords, m.nodeCoords.data(), m.nodeCoords.size() * sizeof(double));
+        if (element_vertices)
+            std::memcpy(element_vertices, m.elemVerts.data(), m.elemVerts.size() * sizeof(int64_t));
+        if (face_side) std::memcpy(face_side, m.faceSide.data(), m.faceSide.size());
+    } catch (const std::exception&) {
+        return DG_ERR_INVALID;
+    }
+    return DG_OK;
+}
+
+dg_status dg_resolve_bc_sides(int32_t dim, int64_t num_elements, const int64_t* element_vertices,
+                              const int8_t* face_side, int32_t num_groups, const uint32_t* group_side_mask,
+                              const int8_t* group_type, int8_t* face_bc) {
+    if ((dim != 2 && dim != 3) || !element_vertices || !face_side || !face_bc || num_groups < 0) return DG_ERR_INVALID;
+    dg::SimplexMesh m;
+    m.dim = dim;
+    m.K = num_elements;
+    m.elemVerts.assign(element_vertices, element_vertices + num_elements * (dim + 1));
+    m.faceSide.assign(face_side, face_side + num_elements * (dim + 1));
+    std::vector<uint32_t> masks(group_side_mask, group_side_mask + num_groups);
+    std::vector<int8_t> types(group_type, group_type + num_groups);
+    std::vector<int8_t> bc = dg::resolveBoundaryTypes(m, masks, types);
+    std::memcpy(face_bc, bc.data(), bc.size());
+    return DG_OK;
+}
+
+dg_status dg_ref_operators(int32_t dim, int32_t order, double* Dw, double* LIFT, int32_t* fmask, double* Nref,
+                           int32_t* lattice) {
+    try {
+        const dg::RefElement& R = dg::refElement(dim, order);
+        if (Dw) std::memcpy(Dw, R.Dw.data(), R.Dw.size() * sizeof(double));
+        if (LIFT) std::memcpy(LIFT, R.LIFT.data(), R.LIFT.size() * sizeof(double));
+        if (fmask)
+            for (size_t i = 0; i < R.fmask.size(); ++i) fmask[i] = R.fmask[i];
+        if (Nref) std::memcpy(Nref, R.Nref.data(), R.Nref.size() * sizeof(double));
+        if (lattice)
+            for (int n = 0; n < R.Np; ++n)
+                for (int k = 0; k < 4; ++k) lattice[n * 4 + k] = R.bary[n][k];
+    } catch (const std::exception&) {
+        return DG_ERR_UNSUPPORTED;
+    }
+    return DG_OK;
+}
+
+dg_status dg_mesh_precompute(const dg_mesh_desc* m, double* Jinv, int32_t* nbr, uint8_t* code, int32_t* face_perm,
+                             int32_t* to_canonical) {
+    if (!m || !m->node_coords || !m->element_vertices) return DG_ERR_INVALID;
+    try {
+        dg::DGMesh g = dg::buildDGMesh(m->dim, m->order, m->num_elements, m->node_coords, m->element_vertices,
+                                       m->face_bc);
+        if (Jinv) std::memcpy(Jinv, g.Jinv.data(), g.Jinv.size() * sizeof(double));
+        if (nbr) std::memcpy(nbr, g.nbr.data(), g.nbr.size() * sizeof(int32_t));
+        if (code) std::memcpy(code, g.nbrCode.data(), g.nbrCode.size());
+        if (to_canonical) std::memcpy(to_canonical, g.toCanon.data(), g.toCanon.size() * sizeof(int32_t));
+        if (face_perm) {
+            const dg::RefElement& R = dg::refElement(m->dim, m->order);
+            for (size_t i = 0; i < R.facePerm.size(); ++i) face_perm[i] = R.facePerm[i];
+        }
+    } catch (const std::exception&) {
+        return DG_ERR_MESH;
+    }
+    return DG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,179 @@
+/*
+ * dgfem_amd — C ABI of the MI355X (gfx950) DG acoustic solver core.
+ *
+ * Drop-in for the time-integration call of the reference DGFEM-Acoustic solver:
+ *   void solver::rungeKutta(std::vector<std::vector<double>>& u, Mesh& mesh, Config config)
+ *   (include/solver.h:25, src/solver.cpp:311-584)
+ * and its sibling solver::forwardEuler (include/solver.h:16, src/solver.cpp:66-302).
+ * Plain pointers and sizes only; status codes instead of exit(); one context per host
+ * thread; the context owns its device memory, stream and graphs.
+ *
+ * State layout at this boundary is the reference's: u[field][element * Np + node] with
+ * fields (p, vx, vy, vz) — plus rho for the 5-field variant — and the element's node order
+ * as in the mesh file (Gmsh element node order; the first dim+1 nodes are the vertices).
+ */
+#ifndef DGFEM_AMD_H
+#define DGFEM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DG_API __attribute__((visibility("default")))
+
+typedef int32_t dg_status;
+enum {
+    DG_OK = 0,
+    DG_ERR_INVALID = 1,     /* bad argument */
+    DG_ERR_MESH = 2,        /* malformed / unsupported mesh */
+    DG_ERR_DEVICE = 3,      /* HIP runtime error or no device */
+    DG_ERR_NONFINITE = 4,   /* non-finite residual detected during dg_run */
+    DG_ERR_UNSUPPORTED = 5, /* order / dimension / option not built */
+    DG_ERR_CALLBACK = 6     /* a caller callback returned nonzero */
+};
+
+enum { DG_FP64 = 0, DG_MIXED = 1 };                 /* precision */
+enum { DG_FIELDS_REF4 = 0, DG_FIELDS_LEE5 = 1 };    /* (p,vx,vy,vz) or + rho */
+enum { DG_BC_ABSORBING = 0, DG_BC_REFLECTING = 1 }; /* src/Mesh.cpp:575-578 */
+enum { DG_RK4 = 0, DG_EULER = 1 };                  /* config timeIntMethod */
+
+typedef struct dg_context dg_context;
+
+typedef struct {
+    int32_t dim;                       /* 2 or 3 */
+    int32_t order;                     /* Lagrange order p (1..5) */
+    int64_t num_elements;              /* K */
+    int32_t nodes_per_element;         /* Np, must equal the order's node count */
+    const double* node_coords;         /* [K][Np][3] */
+    const int64_t* element_vertices;   /* [K][dim+1] vertex ids (face matching) */
+    const int8_t* face_bc;             /* [K][dim+1] BC of boundary faces per Gmsh local face
+                                          (tri edges {0,1},{1,2},{2,0}; tet faces {0,2,1},{0,1,3},
+                                          {0,3,2},{3,1,2}); NULL: all absorbing (reference default) */
+} dg_mesh_desc;
+
+typedef struct {
+    double v0[3];    /* mean flow */
+    double rho0;     /* mean density */
+    double c0;       /* speed of sound */
+    int32_t fields;  /* DG_FIELDS_REF4 | DG_FIELDS_LEE5 */
+} dg_physics;
+
+typedef struct {
+    int32_t precision;        /* DG_FP64 | DG_MIXED */
+    int32_t use_graph;        /* 1: replay captured hipGraphs per batch of steps */
+    int32_t steps_per_graph;  /* steps captured per graph (>= 1) */
+    int32_t device;           /* HIP device ordinal */
+} dg_options;
+
+/* Mesh::Mesh + Mesh::precomputeMassMatrix (src/Mesh.cpp:20-724): FP64 precompute on the
+   host, upload of the device-resident operators and geometry. */
+DG_API dg_status dg_create(const dg_mesh_desc* mesh, const dg_physics* phys, const dg_options* opt,
+                           dg_context** out);
+DG_API void dg_destroy(dg_context* ctx);
+/* Message of the last failure on ctx; with ctx == NULL, of the last failed dg_create. */
+DG_API const char* dg_last_error(const dg_context* ctx);
+
+/* u = [fields][K*Np] (src/dgalerkin.cpp:51). */
+DG_API dg_status dg_set_state(dg_context* ctx, const double* u);
+DG_API dg_status dg_get_state(dg_context* ctx, double* u);
+
+/* Initial condition p += amp * exp(-|x - xc|^2 / size) at every DG node
+   (src/dgalerkin.cpp:51-70); for the 5-field variant rho += that / c0^2. */
+DG_API dg_status dg_add_gaussian(dg_context* ctx, double x, double y, double z, double size, double amp);
+
+/* Hard sources (src/solver.cpp:340-358 node search, :455-483 overwrite). Node set:
+   DG nodes with |x - xs|^2 < size^2. Returns the number of nodes through *num_nodes. */
+DG_API dg_status dg_add_source_monopole(dg_context* ctx, double x, double y, double z, double size, double amp,
+                                        double freq, double phase, double duration, int64_t* num_nodes);
+/* Formula / file sources: value(user, t, &v) is evaluated on the host for every step;
+   it returns nonzero when the pressure at the source nodes is to be overwritten with v. */
+typedef int (*dg_value_fn)(void* user, double t, double* value);
+DG_API dg_status dg_add_source_table(dg_context* ctx, double x, double y, double z, double size, dg_value_fn fn,
+                                     void* user, int64_t* num_nodes);
+
+/* Observer (src/solver.cpp:361-383, :552-573): DG nodes with |x - xo| < size, inverse
+   distance weights 1/(d^2 + 1e-12). */
+DG_API dg_status dg_add_observer(dg_context* ctx, double x, double y, double z, double size, int64_t* num_nodes);
+
+typedef struct {
+    double t_start, t_end, dt, t_rate;  /* the loop of src/solver.cpp:413-414 */
+    int32_t integrator;                 /* DG_RK4 | DG_EULER */
+    int32_t compute_residuals;          /* 1: per-step residuals vs the last snapshot */
+    int64_t max_steps;                  /* stop after this many steps (<0: no limit) */
+} dg_run_params;
+
+typedef struct {
+    int64_t step;
+    double t;                /* label of the step (time before the update) */
+    const double* residuals; /* [5]: p, rho, vx, vy, vz */
+    int32_t num_observers;
+    const double* observers; /* [num_observers][4]: p, vx, vy, vz */
+} dg_step_record;
+
+typedef int (*dg_step_fn)(void* user, const dg_step_record* rec);
+/* Save steps (src/solver.cpp:421-453): state [fields][K*Np] before the source overwrite. */
+typedef int (*dg_save_fn)(void* user, int64_t step, double t, const double* state);
+
+/* solver::rungeKutta / solver::forwardEuler time loop. Callbacks may be NULL. */
+DG_API dg_status dg_run(dg_context* ctx, const dg_run_params* prm, dg_step_fn on_step, dg_save_fn on_save,
+                        void* user, int64_t* steps_done);
+
+/* Convenience: run and write times [max], residuals [max][5], observers [max][nobs][4]. */
+DG_API dg_status dg_run_collect(dg_context* ctx, const dg_run_params* prm, double* times, double* residuals,
+                                double* observers, int64_t capacity, int64_t* steps_done);
+
+/* One right-hand side: k = dt * M^-1 (S - F) for state u, i.e. numStep after updateFlux
+   with beta = 0 (src/solver.cpp:38-57). u and k are [fields][K*Np]. */
+DG_API dg_status dg_rhs(dg_context* ctx, const double* u, double* k, double dt);
+
+typedef struct {
+    int32_t dim, order, nodes_per_element, nodes_per_face, fields, precision;
+    int64_t num_elements, num_faces, num_boundary_faces;
+    double min_inradius;
+    int64_t device_bytes;
+} dg_info;
+DG_API dg_status dg_get_info(const dg_context* ctx, dg_info* info);
+
+/* Per-kernel device time of `steps` eager RK4 steps measured with HIP events on the
+   context's stream: stage_ms[4] (mean per launch of stages 1..4), source_ms, post_ms. */
+DG_API dg_status dg_profile_steps(dg_context* ctx, double dt, int32_t steps, double* stage_ms, double* source_ms,
+                                  double* post_ms);
+
+/* Wall time in ms (HIP events) of the device work of the last dg_run. */
+DG_API double dg_last_run_device_ms(const dg_context* ctx);
+
+/* ---- host-side mesh helpers (no device needed) ---- */
+
+/* Seeded jittered box mesh on [lo,hi]^dim (n cells per side; triangles or Kuhn tets).
+   Sizes first, then fill caller-allocated arrays:
+     node_coords [K][Np][3], element_vertices [K][dim+1], face_side [K][dim+1]
+   (face_side: -1 interior, else side 2*axis + (0: lo, 1: hi)). */
+DG_API dg_status dg_box_mesh_size(int32_t dim, int32_t n, int32_t order, int64_t* num_elements,
+                                  int32_t* nodes_per_element);
+DG_API dg_status dg_box_mesh_fill(int32_t dim, int32_t n, int32_t order, double jitter, uint64_t seed, double lo,
+                                  double hi, double* node_coords, int64_t* element_vertices, int8_t* face_side);
+
+/* The reference's boundary-type rule (src/Mesh.cpp:579-605) over physical groups given as
+   side masks (ascending tag order): face_bc[K][dim+1]. */
+DG_API dg_status dg_resolve_bc_sides(int32_t dim, int64_t num_elements, const int64_t* element_vertices,
+                                     const int8_t* face_side, int32_t num_groups, const uint32_t* group_side_mask,
+                                     const int8_t* group_type, int8_t* face_bc);
+
+/* Diagnostics of the host precompute (no device needed), for tests of the host logic.
+   Reference operators of (dim, p) in the canonical lattice order:
+     Dw [dim][Np][Np] weak derivatives, LIFT [Np][(dim+1)*Nfp], fmask [(dim+1)][Nfp],
+     Nref [(dim+1)][3], lattice [Np][4] integer barycentrics. Any pointer may be NULL. */
+DG_API dg_status dg_ref_operators(int32_t dim, int32_t order, double* Dw, double* LIFT, int32_t* fmask, double* Nref,
+                                  int32_t* lattice);
+/* Mesh precompute (src/Mesh.cpp:20-686 counterpart): Jinv [K][3][3], neighbour [K][dim+1]
+   (-1-bc on the boundary), neighbour code [K][dim+1] (local face | perm << 2),
+   face_perm [nperm][Nfp], to_canonical [K][Np]. Any output pointer may be NULL. */
+DG_API dg_status dg_mesh_precompute(const dg_mesh_desc* mesh, double* Jinv, int32_t* neighbour, uint8_t* code,
+                                    int32_t* face_perm, int32_t* to_canonical);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DGFEM_AMD_H */
