@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -55,13 +56,20 @@ struct dg_context {
     std::vector<TableSource> srcTables;            // per source (fn null for monopoles)
     std::vector<std::vector<int64_t>> obsNodes;
     std::vector<std::vector<double>> obsW;
-    int64_t *dSrcOff = nullptr, *dSrcIdx = nullptr, *dObsOff = nullptr, *dObsIdx = nullptr;
+    int64_t *dSrcOff = nullptr, *dSrcIdx = nullptr, *dObsOff = nullptr, *dObsIdx = nullptr, *dSegBegin = nullptr;
+    int32_t *dSegObs = nullptr, *dObsSeg0 = nullptr;
+    double* dSegPartial = nullptr;
+    int nseg = 0;
     double *dSrcParams = nullptr, *dObsW = nullptr;
     double *dT = nullptr, *dV = nullptr;
     uint8_t* dVF = nullptr;
     double *resRing = nullptr, *obsRing = nullptr;
     int* nonfinite = nullptr;
     int64_t* counter = nullptr;
+    int64_t* batchSteps = nullptr;
+    unsigned int* ticket = nullptr;
+    double* resPartial2 = nullptr;
+    int64_t postBlocks = 1;
     int64_t ringSlots = 0;
     // pinned staging
     double *hT = nullptr, *hV = nullptr, *hRes = nullptr, *hObs = nullptr;
@@ -72,6 +80,10 @@ struct dg_context {
     int gSteps = 0, gIntegrator = -1, gResid = -1;
     // dispatch
     StageFn stageFn = nullptr;
+    int eb = 1;              // elements per block of the chosen stage kernel
+    bool mfma = false;       // matrix-core stage kernel in use
+    void* afrag = nullptr;   // A-operand fragments of the matrix-core kernel
+    std::vector<double> afragHost;
     PerStepFn sourceFn = nullptr, postFn = nullptr;
     int64_t deviceBytes = 0;
     double lastRunMs = 0.0;
@@ -141,9 +153,7 @@ void launchStage(dg_context* c, int stage, const void* qin, void* qout, void* uo
                        dim3(dgk::kBlock), 0, c->stream, a);
 }
 
-template <typename T, bool COMP>
-void launchSource(dg_context* c) {
-    if (c->srcNodes.empty()) return;
+dgk::SourceArgs sourceArgs(dg_context* c) {
     dgk::SourceArgs s{};
     s.nodeOffsets = c->dSrcOff;
     s.nodeIdx = c->dSrcIdx;
@@ -151,12 +161,69 @@ void launchSource(dg_context* c) {
     s.tTable = c->dT;
     s.vTable = c->dV;
     s.vFlag = c->dVF;
-    s.stepCounter = c->counter;
     s.nsrc = static_cast<int>(c->srcNodes.size());
     s.nf = c->nf;
-    hipLaunchKernelGGL((dgk::sourceKernel<T, COMP>), dim3(1, static_cast<unsigned>(std::min<size_t>(c->srcNodes.size(), 64))),
-                       dim3(64), 0, c->stream, s, static_cast<T*>(c->u), static_cast<T*>(COMP ? c->ulo : nullptr),
-                       1.0 / (c->phys.c0 * c->phys.c0));
+    s.ic2 = 1.0 / (c->phys.c0 * c->phys.c0);
+    return s;
+}
+
+// Source overwrite of the first step of a batch; later steps are overwritten by the
+// previous step's post kernel.
+template <int DIM, int P, typename T, bool COMP>
+void launchStageWave(dg_context* c, int stage, const void* qin, void* qout, void* uout, bool snap) {
+    dgk::StageArgs<T> a{};
+    a.qin = static_cast<const T*>(qin);
+    a.qout = static_cast<T*>(qout);
+    a.u = static_cast<T*>(c->u);
+    a.uout = static_cast<T*>(uout);
+    a.ulo = COMP ? static_cast<T*>(c->ulo) : nullptr;
+    a.acc = static_cast<T*>(c->acc);
+    a.geo = static_cast<const T*>(c->geo);
+    a.nbr = c->nbr;
+    a.code = c->code;
+    a.ops = static_cast<const T*>(c->ops);
+    a.iops = c->iops;
+    a.snap = snap ? static_cast<const T*>(c->snap) : nullptr;
+    a.snaplo = (snap && COMP) ? static_cast<const T*>(c->snaplo) : nullptr;
+    a.resPartial = c->resPartial;
+    a.K = c->K;
+    a.dt = static_cast<T>(c->curDt);
+    a.stage = stage;
+    a.ph = makePhysics<T>(c->phys);
+    hipLaunchKernelGGL((dgk::stageKernelWave<DIM, P, T, COMP>), dim3(static_cast<unsigned>(c->nblocks)),
+                       dim3(dgk::kBlock), 0, c->stream, a, static_cast<const T*>(c->afrag));
+}
+
+// A-operand fragments [MT][KSV+KSL][64 lanes] of the matrix-core kernel: row 16*mt + lane%16,
+// k = 4*ks + lane/16 of [Dw_1 | .. | Dw_dim] (volume k-steps) then of -LIFT (lift k-steps).
+template <int DIM, int P>
+void buildAfrag(dg_context* c, const dg::RefElement& R) {
+    using M = dgk::MfmaShape<DIM, P>;
+    const int Np = R.Np, NFT = R.Nfaces * R.Nfp;
+    c->afragHost.assign(M::kFragTotal, 0.0);
+    for (int mt = 0; mt < M::MT; ++mt)
+        for (int ks = 0; ks < M::KS; ++ks)
+            for (int lane = 0; lane < 64; ++lane) {
+                const int row = 16 * mt + lane % 16;
+                const int k = 4 * ks + lane / 16;
+                double v = 0.0;
+                if (row < Np) {
+                    if (ks < M::KSV) {
+                        if (k < M::KV) v = R.Dw[(static_cast<size_t>(k / Np) * Np + row) * Np + k % Np];
+                    } else {
+                        const int fa = 4 * (ks - M::KSV) + lane / 16;
+                        if (fa < NFT) v = -R.LIFT[static_cast<size_t>(row) * NFT + fa];
+                    }
+                }
+                c->afragHost[(mt * M::KS + ks) * 64 + lane] = v;
+            }
+}
+
+template <typename T, bool COMP>
+void launchSource(dg_context* c) {
+    if (c->srcNodes.empty()) return;
+    hipLaunchKernelGGL((dgk::sourceKernel<T, COMP>), dim3(1), dim3(256), 0, c->stream, sourceArgs(c), c->counter,
+                       static_cast<T*>(c->u), static_cast<T*>(COMP ? c->ulo : nullptr));
 }
 
 template <typename T, bool COMP>
@@ -164,23 +231,53 @@ void launchPost(dg_context* c) {
     dgk::PostArgs p{};
     p.resPartial = c->resPartial;
     p.nblocks = c->nblocks;
+    p.resPartial2 = c->resPartial2;
+    p.ticket = c->ticket;
     p.invN = 1.0 / static_cast<double>(c->K * c->Np);
     p.obsOffsets = c->dObsOff;
     p.obsIdx = c->dObsIdx;
     p.obsW = c->dObsW;
+    p.segBegin = c->dSegBegin;
+    p.segObs = c->dSegObs;
+    p.obsSeg0 = c->dObsSeg0;
+    p.segPartial = c->dSegPartial;
+    p.nseg = c->nseg;
     p.nobs = static_cast<int>(c->obsNodes.size());
     p.nf = c->nf;
     p.resRing = c->resRing;
     p.obsRing = c->obsRing;
     p.nonfinite = c->nonfinite;
     p.stepCounter = c->counter;
+    p.batchSteps = c->batchSteps;
     p.computeResiduals = c->curResid;
-    hipLaunchKernelGGL((dgk::postKernel<T, COMP>), dim3(1), dim3(256), 0, c->stream, p,
-                       static_cast<const T*>(c->u), static_cast<const T*>(COMP ? c->ulo : nullptr));
+    hipLaunchKernelGGL((dgk::postKernel<T, COMP>), dim3(static_cast<unsigned>(c->postBlocks)), dim3(256), 0, c->stream,
+                       p, sourceArgs(c), static_cast<T*>(c->u), static_cast<T*>(COMP ? c->ulo : nullptr));
 }
 
 template <int DIM, int P, int NF>
 bool pick(dg_context* c) {
+    // Stage-kernel variant: DGFEM_KERNEL = valu (default) | wave (matrix-core, 4 fields).
+    const char* env = std::getenv("DGFEM_KERNEL");
+    const std::string kind = env ? env : "valu";
+    if constexpr (NF == 4 && dgk::MfmaShape<DIM, P>::kSupported) {
+        if (kind == "wave") {
+            c->mfma = true;
+            buildAfrag<DIM, P>(c, dg::refElement(DIM, P));
+            c->eb = dgk::WaveShape<DIM, P>::EB;
+            if (c->precision == DG_FP64) {
+                c->stageFn = &launchStageWave<DIM, P, double, false>;
+                c->sourceFn = &launchSource<double, false>;
+                c->postFn = &launchPost<double, false>;
+            } else {
+                c->stageFn = &launchStageWave<DIM, P, float, true>;
+                c->sourceFn = &launchSource<float, true>;
+                c->postFn = &launchPost<float, true>;
+            }
+            return true;
+        }
+    }
+    c->mfma = false;
+    c->eb = dgk::Sizes<DIM, P>::EB;
     if (c->precision == DG_FP64) {
         c->stageFn = &launchStage<DIM, P, NF, double, false>;
         c->sourceFn = &launchSource<double, false>;
@@ -254,6 +351,13 @@ void freeTables(dg_context* c) {
     devFree(c->dObsOff);
     devFree(c->dObsIdx);
     devFree(c->dObsW);
+    devFree(c->dSegBegin);
+    devFree(c->dSegObs);
+    devFree(c->dObsSeg0);
+    devFree(c->dSegPartial);
+    c->dSegBegin = nullptr;
+    c->dSegObs = c->dObsSeg0 = nullptr;
+    c->dSegPartial = nullptr;
     c->dSrcOff = c->dSrcIdx = c->dObsOff = c->dObsIdx = nullptr;
     c->dSrcParams = c->dObsW = nullptr;
 }
@@ -289,6 +393,34 @@ void syncTables(dg_context* c) {
     c->dObsOff = uploadVec(c, ooff);
     c->dObsIdx = uploadVec(c, oidx);
     c->dObsW = uploadVec(c, ow);
+    // observer node lists split into segments of at most 256 entries
+    std::vector<int64_t> segBegin;
+    std::vector<int32_t> segObs, obsSeg0{0};
+    for (size_t o = 0; o < c->obsNodes.size(); ++o) {
+        for (int64_t b = ooff[o]; b < ooff[o + 1]; b += 256) {
+            segBegin.push_back(b);
+            segObs.push_back(static_cast<int32_t>(o));
+        }
+        obsSeg0.push_back(static_cast<int32_t>(segBegin.size()));
+    }
+    c->nseg = static_cast<int>(segBegin.size());
+    segBegin.push_back(oidx.size());
+    // a segment ends where the next begins or at the end of its observer's list
+    std::vector<int64_t> segEndFix(segBegin);
+    for (int sgi = 0; sgi < c->nseg; ++sgi) {
+        int o = segObs[sgi];
+        segEndFix[sgi + 1] = std::min<int64_t>(segBegin[sgi] + 256, ooff[o + 1]);
+    }
+    // store begin/end pairs as consecutive entries: begin[s], end[s] == begin of the next slot
+    std::vector<int64_t> segBE(2 * std::max(1, c->nseg));
+    for (int sgi = 0; sgi < c->nseg; ++sgi) {
+        segBE[2 * sgi] = segBegin[sgi];
+        segBE[2 * sgi + 1] = segEndFix[sgi + 1];
+    }
+    c->dSegBegin = uploadVec(c, segBE);
+    c->dSegObs = uploadVec(c, segObs);
+    c->dObsSeg0 = uploadVec(c, obsSeg0);
+    c->dSegPartial = devAlloc<double>(c, std::max(1, c->nseg) * 5);
     // rings and per-step tables
     const int64_t ns = std::max<int64_t>(1, static_cast<int64_t>(c->srcNodes.size()));
     const int64_t no = std::max<int64_t>(1, static_cast<int64_t>(c->obsNodes.size()));
@@ -313,7 +445,6 @@ void syncTables(dg_context* c) {
 
 // Launch the kernels of one step on the context stream.
 void launchStep(dg_context* c, int integrator, bool resid) {
-    c->sourceFn(c);
     if (integrator == DG_RK4) {
         c->stageFn(c, dgk::kStageRK1, c->u, c->qa, c->u, false);
         c->stageFn(c, dgk::kStageRK2, c->qa, c->qb, c->u, false);
@@ -403,7 +534,7 @@ void destroyContext(dg_context* c) {
     if (c->device >= 0) (void)hipSetDevice(c->device);
     releaseGraphs(c);
     freeTables(c);
-    for (void* p : {c->u, c->ulo, c->qa, c->qb, c->acc, c->snap, c->snaplo, c->geo, c->ops})
+    for (void* p : {c->u, c->ulo, c->qa, c->qb, c->acc, c->snap, c->snaplo, c->geo, c->ops, c->afrag})
         devFree(p);
     devFree(c->nbr);
     devFree(c->iops);
@@ -416,6 +547,9 @@ void destroyContext(dg_context* c) {
     devFree(c->obsRing);
     devFree(c->nonfinite);
     devFree(c->counter);
+    devFree(c->batchSteps);
+    devFree(c->ticket);
+    devFree(c->resPartial2);
     for (void* h : {static_cast<void*>(c->hT), static_cast<void*>(c->hV), static_cast<void*>(c->hRes),
                     static_cast<void*>(c->hObs), static_cast<void*>(c->hVF)})
         if (h) (void)hipHostFree(h);
@@ -482,8 +616,7 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         c->device = c->opt.device;
         HIPCHK(hipSetDevice(c->device));
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        const int EB = dgk::kBlock / c->Np;
-        c->nblocks = (c->K + EB - 1) / EB;
+        c->nblocks = (c->K + c->eb - 1) / c->eb;
         const size_t nstate = static_cast<size_t>(c->K) * c->Np * c->nf;
         auto alloc = [&](size_t n) -> void* {
             return c->precision == DG_FP64 ? static_cast<void*>(devAlloc<double>(c, n))
@@ -510,12 +643,15 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         std::vector<int32_t> iops;
         buildOps(R, ops, iops);
         c->ops = alloc(ops.size());
+        if (c->mfma) c->afrag = alloc(c->afragHost.size());
         if (c->precision == DG_FP64) {
             uploadAs<double>(c, c->geo, geo);
             uploadAs<double>(c, c->ops, ops);
+            if (c->mfma) uploadAs<double>(c, c->afrag, c->afragHost);
         } else {
             uploadAs<float>(c, c->geo, geo);
             uploadAs<float>(c, c->ops, ops);
+            if (c->mfma) uploadAs<float>(c, c->afrag, c->afragHost);
         }
         c->iops = uploadVec(c, iops);
         c->nbr = uploadVec(c, c->mesh.nbr);
@@ -526,6 +662,11 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         c->resRing = devAlloc<double>(c, c->ringSlots * 5);
         c->nonfinite = devAlloc<int>(c, 1);
         c->counter = devAlloc<int64_t>(c, 1);
+        c->batchSteps = devAlloc<int64_t>(c, 1);
+        c->ticket = devAlloc<unsigned int>(c, 1);
+        HIPCHK(hipMemset(c->ticket, 0, sizeof(unsigned int)));
+        c->postBlocks = std::max<int64_t>(1, std::min<int64_t>(128, c->nblocks / 128));
+        c->resPartial2 = devAlloc<double>(c, static_cast<size_t>(c->postBlocks) * 5);
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hT), sizeof(double) * c->ringSlots));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hRes), sizeof(double) * c->ringSlots * 5));
         c->tablesDirty = true;
@@ -722,9 +863,10 @@ dg_status dg_run(dg_context* c, const dg_run_params* prm, dg_step_fn on_step, dg
                 HIPCHK(hipMemcpyAsync(c->dV, c->hV, sizeof(double) * n * ns, hipMemcpyHostToDevice, c->stream));
                 HIPCHK(hipMemcpyAsync(c->dVF, c->hVF, n * ns, hipMemcpyHostToDevice, c->stream));
             }
-            hipLaunchKernelGGL(dgk::setCounterKernel, dim3(1), dim3(1), 0, c->stream, c->counter, int64_t(0),
-                               c->nonfinite);
+            hipLaunchKernelGGL(dgk::setCounterKernel, dim3(1), dim3(1), 0, c->stream, c->counter, c->batchSteps,
+                               n, c->nonfinite);
             HIPCHK(hipEventRecord(ev0, c->stream));
+            c->sourceFn(c);
             if (useGraph) {
                 int64_t s = 0;
                 if (c->gMulti)
@@ -851,7 +993,8 @@ dg_status dg_profile_steps(dg_context* c, double dt, int32_t steps, double* stag
         std::vector<double> tt(std::min<int64_t>(steps, c->ringSlots), 0.0);
         for (size_t s = 0; s < tt.size(); ++s) c->hT[s] = s * dt;
         HIPCHK(hipMemcpyAsync(c->dT, c->hT, sizeof(double) * tt.size(), hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(dgk::setCounterKernel, dim3(1), dim3(1), 0, c->stream, c->counter, int64_t(0), c->nonfinite);
+        hipLaunchKernelGGL(dgk::setCounterKernel, dim3(1), dim3(1), 0, c->stream, c->counter, c->batchSteps,
+                           static_cast<int64_t>(tt.size()), c->nonfinite);
         std::vector<hipEvent_t> ev(14);
         for (auto& e : ev) HIPCHK(hipEventCreate(&e));
         double acc[6] = {0, 0, 0, 0, 0, 0};

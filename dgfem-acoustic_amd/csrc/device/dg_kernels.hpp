@@ -164,6 +164,27 @@ __global__ __launch_bounds__(kBlock) void stageKernel(StageArgs<T> a) {
     const int64_t e0 = blk * EB;
     const int nE = static_cast<int>((a.K - e0) < EB ? (a.K - e0) : EB);
 
+    // prefetch the RK operands of this thread's node (consumed after the contractions)
+    const bool active = tid < nE * NP;
+    const int64_t myBase = (e0 * NP + tid) * NF;
+    const bool needU = a.stage == kStageRK2 || a.stage == kStageRK3 || a.stage == kStageRK4;
+    const bool isFinal = a.stage == kStageRK4 || a.stage == kStageEuler;
+    T pu[NF], pacc[NF], plo[NF], ps[NF], psl[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+        pu[k] = pacc[k] = plo[k] = ps[k] = psl[k] = T(0);
+        if (active) {
+            if (needU) {
+                pu[k] = a.u[myBase + k];
+                pacc[k] = a.acc[myBase + k];
+            }
+            if constexpr (COMP) plo[k] = a.ulo[myBase + k];
+            if (isFinal && a.snap) {
+                ps[k] = a.snap[myBase + k];
+                if constexpr (COMP) psl[k] = a.snaplo[myBase + k];
+            }
+        }
+    }
     if constexpr (kOpsInLds)
         for (int x = tid; x < L::kTotal; x += kBlock) sOps[x] = a.ops[x];
     for (int x = tid; x < L::kITotal; x += kBlock) sI[x] = a.iops[x];
@@ -291,17 +312,16 @@ __global__ __launch_bounds__(kBlock) void stageKernel(StageArgs<T> a) {
 #pragma unroll
             for (int k = 0; k < NF; ++k) {
                 const T kk = dt * rhs[k];
-                T uv, ul = T(0);
+                T uv;
                 if (a.stage == kStageRK1) {
                     uv = q0[i * NF + k];
                     a.acc[base + k] = kk;
                 } else {
-                    uv = a.u[base + k];
-                    a.acc[base + k] = a.acc[base + k] + T(2) * kk;
+                    uv = pu[k];
+                    a.acc[base + k] = pacc[k] + T(2) * kk;
                 }
-                if constexpr (COMP) ul = a.ulo[base + k];
                 if constexpr (COMP)
-                    a.qout[base + k] = uv + (ul + half * kk);
+                    a.qout[base + k] = uv + (plo[k] + half * kk);
                 else
                     a.qout[base + k] = uv + half * kk;
             }
@@ -312,12 +332,11 @@ __global__ __launch_bounds__(kBlock) void stageKernel(StageArgs<T> a) {
 #pragma unroll
             for (int k = 0; k < NF; ++k) {
                 const T kk = dt * rhs[k];
-                const T delta = (a.stage == kStageRK4) ? (a.acc[base + k] + kk) / T(6) : kk;
-                const T uv = (a.stage == kStageRK4) ? a.u[base + k] : q0[i * NF + k];
+                const T delta = (a.stage == kStageRK4) ? (pacc[k] + kk) / T(6) : kk;
+                const T uv = (a.stage == kStageRK4) ? pu[k] : q0[i * NF + k];
                 if constexpr (COMP) {
                     T s, e, t, e2;
-                    const T ul = a.ulo[base + k];
-                    twoSum(ul, delta, t, e2);
+                    twoSum(plo[k], delta, t, e2);
                     twoSum(uv, t, s, e);
                     const T nlo = e + e2;
                     T h, l2;
@@ -335,20 +354,16 @@ __global__ __launch_bounds__(kBlock) void stageKernel(StageArgs<T> a) {
             }
             if (a.snap) {
                 double d[4];
+                double sv[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    double sv = static_cast<double>(a.snap[base + k]);
-                    if constexpr (COMP) sv += static_cast<double>(a.snaplo[base + k]);
-                    d[k] = (static_cast<double>(unew[k]) + lo[k]) - sv;
+                    sv[k] = static_cast<double>(ps[k]);
+                    if constexpr (COMP) sv[k] += static_cast<double>(psl[k]);
+                    d[k] = (static_cast<double>(unew[k]) + lo[k]) - sv[k];
                 }
                 const double ic2 = static_cast<double>(a.ph.ic2);
-                double sp = 0.0;
-                {
-                    double s0 = static_cast<double>(a.snap[base]);
-                    if constexpr (COMP) s0 += static_cast<double>(a.snaplo[base]);
-                    double u0 = static_cast<double>(unew[0]) + lo[0];
-                    sp = s0 * ic2 - u0 * ic2;
-                }
+                const double u0 = static_cast<double>(unew[0]) + lo[0];
+                const double sp = sv[0] * ic2 - u0 * ic2;
                 res[0] = d[0] * d[0];
                 res[1] = sp * sp;
                 res[2] = d[1] * d[1];
@@ -374,6 +389,366 @@ __global__ __launch_bounds__(kBlock) void stageKernel(StageArgs<T> a) {
     }
 }
 
+// ------------------------------------------------------------ MFMA building blocks ----
+//
+// Matrix-core form of the two contractions of a stage (4 fields):
+//   rhs = [Dw_1 | .. | Dw_DIM] (NP x DIM*NP) . G  +  (-LIFT) (NP x NFT) . (Fscale pi)
+// with G_r = sum_x (dxi_r/dx_x) F_x(q). Columns of the B operand are (4 elements x 4
+// fields). FP32 uses v_mfma_f32_16x16x4f32 (D row = 4*(lane/16)+r), FP64
+// v_mfma_f64_16x16x4f64 (D row = lane/16 + 4*r); A lane = row + 16*k, B lane = col + 16*k
+// for both (layouts checked on gfx950).
+
+typedef float mf4 __attribute__((ext_vector_type(4)));
+typedef double md4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct MfmaTraits;
+template <>
+struct MfmaTraits<float> {
+    using V = mf4;
+    __device__ static V mma(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+template <>
+struct MfmaTraits<double> {
+    using V = md4;
+    __device__ static V mma(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+template <int DIM, int P>
+struct MfmaShape {
+    using S = Sizes<DIM, P>;
+    static constexpr int MT = (S::NP + 15) / 16;       // row tiles
+    static constexpr int KV = DIM * S::NP;              // volume contraction length
+    static constexpr int KSV = (KV + 3) / 4;            // k-steps (volume)
+    static constexpr int KSL = (S::NFT + 3) / 4;        // k-steps (lift)
+    static constexpr int KS = KSV + KSL;
+    static constexpr bool kSupported = MT * KS <= 64;   // A fragments fit in registers
+    static constexpr int kFragTotal = MT * KS * 64;
+};
+
+// ------------------------------------------------- wave-autonomous MFMA stage kernel ----
+//
+// Each wave owns groups of 4 elements and runs the whole stage for them without block-wide
+// barriers: state to wave-private LDS, face normals, numerical fluxes, then one matrix-core
+// accumulation rhs = [Dw_1|..|Dw_DIM] . G + (-LIFT) . (Fscale pi) where the B operand G is
+// folded on the fly from the staged state (G_r = sum_x (dxi_r/dx_x) F_x(q)), then an LDS
+// transpose so that the RK register update is done with coalesced 16-byte accesses.
+// The A fragments (constant operators) are staged once per block in LDS.
+
+__device__ inline void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+    using V = float4;
+};
+template <>
+struct Vec4<double> {
+    using V = double4;
+};
+
+template <int DIM, int P>
+struct WaveShape {
+    using M = MfmaShape<DIM, P>;
+    static constexpr int NG = 2;            // element groups (of 4) per wave
+    static constexpr int EB = 4 * 4 * NG;   // elements per block
+};
+
+template <int DIM, int P, typename T, bool COMP>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void stageKernelWave(
+    StageArgs<T> a, const T* __restrict__ afrag) {
+    using S = Sizes<DIM, P>;
+    using L = OpLayout<DIM, P>;
+    using M = MfmaShape<DIM, P>;
+    using W = WaveShape<DIM, P>;
+    using MT_ = MfmaTraits<T>;
+    using V4 = typename Vec4<T>::V;
+    constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC, NFT = S::NFT, NF = 4;
+    constexpr int JJ = DIM * DIM;
+    constexpr int NCH = (4 * NP + 63) / 64;  // float4 chunks (nodes) per lane per group
+
+    __shared__ T sA[M::kFragTotal];
+    __shared__ int32_t sI[L::kITotal];
+    __shared__ T sNref[NFC * 3];
+    __shared__ __align__(16) T sQ[4][4 * NP * NF];    // per wave: staged state of its group
+    __shared__ __align__(16) T sF[4][M::KSL * 4 * 16];  // per wave: lift B rows (fa x 16 cols)
+    __shared__ __align__(16) T sR[4][4 * NP * NF];    // per wave: rhs transpose
+    __shared__ T sJ[4][4 * JJ];
+    __shared__ T sN[4][4 * NFC * 4];
+    __shared__ double sRed[kBlock / 64][5];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t blk = xcdRemap(blockIdx.x, gridDim.x);
+    const int stage = a.stage;
+    const bool needU = stage == kStageRK2 || stage == kStageRK3 || stage == kStageRK4;
+    const bool isFinal = stage == kStageRK4 || stage == kStageEuler;
+
+    for (int x = tid; x < M::kFragTotal; x += kBlock) sA[x] = afrag[x];
+    for (int x = tid; x < L::kITotal; x += kBlock) sI[x] = a.iops[x];
+    for (int x = tid; x < NFC * 3; x += kBlock) sNref[x] = a.ops[L::kN + x];
+    __syncthreads();
+
+    const int col = lane & 15, cel = col >> 2, fld = col & 3, brow = lane >> 4;
+    T* Q = sQ[wave];
+    T* F = sF[wave];
+    T* R = sR[wave];
+    T* Jw = sJ[wave];
+    T* Nw = sN[wave];
+    double res[5] = {0, 0, 0, 0, 0};
+
+    for (int g = 0; g < W::NG; ++g) {
+        const int64_t e0 = (blk * 4 + wave) * (4 * W::NG) + 4 * g;  // first element of the group
+        if (e0 >= a.K) break;
+        const int nE = static_cast<int>((a.K - e0) < 4 ? (a.K - e0) : 4);
+        const int nNodes = nE * NP;
+        // -- stage the group's state and geometry; prefetch this lane's RK operands
+        const V4* qin4 = reinterpret_cast<const V4*>(a.qin + e0 * NP * NF);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const int node = lane + 64 * c;
+            if (node < nNodes) reinterpret_cast<V4*>(Q)[node] = qin4[node];
+        }
+        for (int x = lane; x < nE * JJ; x += 64) Jw[x] = a.geo[e0 * JJ + x];
+        for (int x = lane; x < M::KSL * 4 * 16; x += 64) F[x] = T(0);
+        waveSync();
+        // -- unit normals and Fscale of the group's faces
+        if (lane < nE * NFC) {
+            const int e = lane / NFC, f = lane - e * NFC;
+            const T* J = &Jw[e * JJ];
+            T nv[3] = {T(0), T(0), T(0)};
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) {
+                T sum = T(0);
+#pragma unroll
+                for (int r = 0; r < DIM; ++r) sum += J[r * DIM + c] * sNref[f * 3 + r];
+                nv[c] = sum;
+            }
+            const T fs = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+            const T ifs = T(1) / fs;
+            Nw[lane * 4 + 0] = nv[0] * ifs;
+            Nw[lane * 4 + 1] = nv[1] * ifs;
+            Nw[lane * 4 + 2] = nv[2] * ifs;
+            Nw[lane * 4 + 3] = fs;
+        }
+        waveSync();
+        // -- numerical fluxes at the group's face nodes -> lift B rows
+        for (int it = lane; it < nE * NFT; it += 64) {
+            const int e = it / NFT, fa = it - e * NFT;
+            const int f = fa / NFP, an = fa - f * NFP;
+            const int64_t eg = e0 + e;
+            const T* n4 = &Nw[(e * NFC + f) * 4];
+            const T nv[3] = {n4[0], n4[1], n4[2]};
+            const T fs = n4[3];
+            const int own = sI[L::kMask + f * NFP + an];
+            T qm[NF], qp[NF], pi[NF];
+            {
+                const V4 v = reinterpret_cast<const V4*>(Q)[e * NP + own];
+                qm[0] = v.x; qm[1] = v.y; qm[2] = v.z; qm[3] = v.w;
+            }
+            const int nb = a.nbr[eg * NFC + f];
+            if (nb >= 0) {
+                const int cd = a.code[eg * NFC + f];
+                const int fp = cd & 3, pc = cd >> 2;
+                const int nn = sI[L::kMask + fp * NFP + sI[L::kPerm + pc * NFP + an]];
+                const V4 v = reinterpret_cast<const V4*>(a.qin)[static_cast<int64_t>(nb) * NP + nn];
+                qp[0] = v.x; qp[1] = v.y; qp[2] = v.z; qp[3] = v.w;
+            } else {
+                qp[0] = qp[1] = qp[2] = qp[3] = T(0);
+            }
+            numericalFlux<DIM, NF>(a.ph, nv, nb, qm, qp, pi);
+            V4 out;
+            out.x = fs * pi[0]; out.y = fs * pi[1]; out.z = fs * pi[2]; out.w = fs * pi[3];
+            reinterpret_cast<V4*>(F)[fa * 4 + e] = out;
+        }
+        // this lane's element: rows of dxi/dx and v0 . row
+        T Jl[DIM][DIM], vr[DIM];
+        const bool colOk = cel < nE;
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) {
+            T s0 = T(0);
+#pragma unroll
+            for (int xx = 0; xx < DIM; ++xx) {
+                Jl[r][xx] = colOk ? Jw[cel * JJ + r * DIM + xx] : T(0);
+                s0 += Jl[r][xx] * a.ph.v0[xx];
+            }
+            vr[r] = s0;
+        }
+        waveSync();
+        // -- matrix-core contraction; B of the volume part folded on the fly
+        typename MT_::V acc[M::MT];
+#pragma unroll
+        for (int mt = 0; mt < M::MT; ++mt) acc[mt] = typename MT_::V{T(0), T(0), T(0), T(0)};
+#pragma unroll 3
+        for (int ks = 0; ks < M::KSV; ++ks) {
+            const int kk = 4 * ks + brow;
+            T b = T(0);
+            if (kk < M::KV && colOk) {
+                const int r = kk / NP, j = kk - r * NP;
+                const V4 q = reinterpret_cast<const V4*>(Q)[cel * NP + j];
+                T jr[DIM];
+                T jv = T(0), vrr = T(0);
+#pragma unroll
+                for (int rr = 0; rr < DIM; ++rr)
+                    if (rr == r) {
+#pragma unroll
+                        for (int xx = 0; xx < DIM; ++xx) jr[xx] = Jl[rr][xx];
+                        vrr = vr[rr];
+                    }
+                jv = jr[0] * q.y;
+                if (DIM > 1) jv += jr[1] * q.z;
+                if (DIM > 2) jv += jr[DIM > 2 ? 2 : 0] * q.w;
+                const T qf = fld == 0 ? q.x : (fld == 1 ? q.y : (fld == 2 ? q.z : q.w));
+                b = vrr * qf;
+                if (fld == 0)
+                    b += a.ph.rc2 * jv;
+                else if (fld - 1 < DIM)
+                    b += jr[fld - 1 < DIM ? fld - 1 : 0] * q.x * a.ph.irho0;
+            }
+#pragma unroll
+            for (int mt = 0; mt < M::MT; ++mt) acc[mt] = MT_::mma(sA[(mt * M::KS + ks) * 64 + lane], b, acc[mt]);
+        }
+#pragma unroll 2
+        for (int ks = 0; ks < M::KSL; ++ks) {
+            const T b = F[(4 * ks + brow) * 16 + col];
+#pragma unroll
+            for (int mt = 0; mt < M::MT; ++mt)
+                acc[mt] = MT_::mma(sA[(mt * M::KS + M::KSV + ks) * 64 + lane], b, acc[mt]);
+        }
+        // -- transpose the rhs to [element][node][field]
+#pragma unroll
+        for (int mt = 0; mt < M::MT; ++mt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * mt + MT_::row(lane, r);
+                if (i < NP) R[(cel * NP + i) * NF + fld] = acc[mt][r];
+            }
+        waveSync();
+        // -- register update, one node (4 fields) per lane and chunk
+        const T dt = a.dt;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const int node = lane + 64 * c;
+            if (node >= nNodes) continue;
+            const int64_t off = e0 * NP + node;
+            const V4 rv = reinterpret_cast<const V4*>(R)[node];
+            const T kk[4] = {dt * rv.x, dt * rv.y, dt * rv.z, dt * rv.w};
+            if (stage == kStageRHS) {
+                V4 o;
+                o.x = kk[0]; o.y = kk[1]; o.z = kk[2]; o.w = kk[3];
+                reinterpret_cast<V4*>(a.qout)[off] = o;
+                continue;
+            }
+            const V4 q0 = reinterpret_cast<const V4*>(Q)[node];
+            const T qv[4] = {q0.x, q0.y, q0.z, q0.w};
+            V4 pu[1], pacc[1], plo[1], ps[1], psl[1];
+            const int cc = c;
+            c = 0;
+            if (needU) {
+                pu[0] = reinterpret_cast<const V4*>(a.u)[off];
+                pacc[0] = reinterpret_cast<const V4*>(a.acc)[off];
+            }
+            if constexpr (COMP) plo[0] = reinterpret_cast<const V4*>(a.ulo)[off];
+            if (isFinal && a.snap) {
+                ps[0] = reinterpret_cast<const V4*>(a.snap)[off];
+                if constexpr (COMP) psl[0] = reinterpret_cast<const V4*>(a.snaplo)[off];
+            }
+            const T uv4[4] = {pu[c].x, pu[c].y, pu[c].z, pu[c].w};
+            const T ac4[4] = {pacc[c].x, pacc[c].y, pacc[c].z, pacc[c].w};
+            T lo4[4] = {T(0), T(0), T(0), T(0)};
+            if constexpr (COMP) {
+                lo4[0] = plo[c].x; lo4[1] = plo[c].y; lo4[2] = plo[c].z; lo4[3] = plo[c].w;
+            }
+            if (stage <= kStageRK3) {
+                const T half = (stage == kStageRK3) ? T(1) : T(0.5);
+                T na[4], nq[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const T uv = (stage == kStageRK1) ? qv[k] : uv4[k];
+                    na[k] = (stage == kStageRK1) ? kk[k] : ac4[k] + T(2) * kk[k];
+                    if constexpr (COMP)
+                        nq[k] = uv + (lo4[k] + half * kk[k]);
+                    else
+                        nq[k] = uv + half * kk[k];
+                }
+                V4 oa, oq;
+                oa.x = na[0]; oa.y = na[1]; oa.z = na[2]; oa.w = na[3];
+                oq.x = nq[0]; oq.y = nq[1]; oq.z = nq[2]; oq.w = nq[3];
+                reinterpret_cast<V4*>(a.acc)[off] = oa;
+                reinterpret_cast<V4*>(a.qout)[off] = oq;
+            } else {
+                T hi[4], lw[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const T delta = (stage == kStageRK4) ? (ac4[k] + kk[k]) / T(6) : kk[k];
+                    const T uv = (stage == kStageRK4) ? uv4[k] : qv[k];
+                    if constexpr (COMP) {
+                        T sum, e1, t, e2;
+                        twoSum(lo4[k], delta, t, e2);
+                        twoSum(uv, t, sum, e1);
+                        twoSum(sum, e1 + e2, hi[k], lw[k]);
+                    } else {
+                        hi[k] = uv + delta;
+                        lw[k] = T(0);
+                    }
+                }
+                V4 oh;
+                oh.x = hi[0]; oh.y = hi[1]; oh.z = hi[2]; oh.w = hi[3];
+                reinterpret_cast<V4*>(a.uout)[off] = oh;
+                if constexpr (COMP) {
+                    V4 ol;
+                    ol.x = lw[0]; ol.y = lw[1]; ol.z = lw[2]; ol.w = lw[3];
+                    reinterpret_cast<V4*>(a.ulo)[off] = ol;
+                }
+                if (a.snap) {
+                    const T s4[4] = {ps[c].x, ps[c].y, ps[c].z, ps[c].w};
+                    T sl4[4] = {T(0), T(0), T(0), T(0)};
+                    if constexpr (COMP) {
+                        sl4[0] = psl[c].x; sl4[1] = psl[c].y; sl4[2] = psl[c].z; sl4[3] = psl[c].w;
+                    }
+                    double d[4], sv0 = 0.0, un0 = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const double sv = static_cast<double>(s4[k]) + static_cast<double>(sl4[k]);
+                        const double un = static_cast<double>(hi[k]) + static_cast<double>(lw[k]);
+                        d[k] = un - sv;
+                        if (k == 0) { sv0 = sv; un0 = un; }
+                    }
+                    const double ic2 = static_cast<double>(a.ph.ic2);
+                    const double dr = sv0 * ic2 - un0 * ic2;
+                    res[0] += d[0] * d[0];
+                    res[1] += dr * dr;
+                    res[2] += d[1] * d[1];
+                    res[3] += d[2] * d[2];
+                    res[4] += d[3] * d[3];
+                }
+            }
+            c = cc;
+        }
+        waveSync();
+    }
+    if (isFinal && a.snap) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            double v = res[k];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0) sRed[wave][k] = v;
+        }
+        __syncthreads();
+        if (tid < 5) {
+            double sum = 0.0;
+            for (int w = 0; w < kBlock / 64; ++w) sum += sRed[w][tid];
+            a.resPartial[static_cast<int64_t>(blockIdx.x) * 5 + tid] = sum;
+        }
+    }
+}
+
 // ---------------------------------------------------------------- per-step kernels ----
 
 struct SourceArgs {
@@ -383,17 +758,18 @@ struct SourceArgs {
     const double* tTable;        // [slots] step times
     const double* vTable;        // [slots][nsrc] table values (kind 1)
     const uint8_t* vFlag;        // [slots][nsrc] 1 = overwrite (kind 1)
-    const int64_t* stepCounter;  // slot of the current step
     int nsrc;
     int nf;
+    double ic2;                  // 1/c0^2 (5-field variant: rho follows p)
 };
 
-// Hard-source overwrite of the pressure at the source nodes (src/solver.cpp:455-483).
+// Hard-source overwrite of the pressure at the source nodes for step `slot`
+// (src/solver.cpp:455-483): monopole amp*sin(2 pi f t + phase) while t < duration; table
+// sources (formula / file, evaluated on the host) when flagged. Executed by one block.
 template <typename T, bool COMP>
-__global__ void sourceKernel(SourceArgs s, T* __restrict__ u, T* __restrict__ ulo, double ic2) {
-    const int64_t slot = *s.stepCounter;
+__device__ inline void applySources(const SourceArgs& s, int64_t slot, T* __restrict__ u, T* __restrict__ ulo) {
     const double t = s.tTable[slot];
-    for (int src = blockIdx.y; src < s.nsrc; src += gridDim.y) {
+    for (int src = 0; src < s.nsrc; ++src) {
         const double* pr = s.params + src * 5;
         double value;
         bool active;
@@ -406,13 +782,13 @@ __global__ void sourceKernel(SourceArgs s, T* __restrict__ u, T* __restrict__ ul
         }
         if (!active) continue;
         const int64_t b = s.nodeOffsets[src], e = s.nodeOffsets[src + 1];
-        for (int64_t x = b + blockIdx.x * blockDim.x + threadIdx.x; x < e; x += gridDim.x * blockDim.x) {
+        for (int64_t x = b + threadIdx.x; x < e; x += blockDim.x) {
             const int64_t base = s.nodeIdx[x] * s.nf;
             const T hi = static_cast<T>(value);
             u[base] = hi;
             if constexpr (COMP) ulo[base] = static_cast<T>(value - static_cast<double>(hi));
             if (s.nf == 5) {
-                const double rv = value * ic2;
+                const double rv = value * s.ic2;
                 const T rh = static_cast<T>(rv);
                 u[base + 4] = rh;
                 if constexpr (COMP) ulo[base + 4] = static_cast<T>(rv - static_cast<double>(rh));
@@ -421,34 +797,100 @@ __global__ void sourceKernel(SourceArgs s, T* __restrict__ u, T* __restrict__ ul
     }
 }
 
+// Source overwrite for the step held in the step counter (first step of a batch).
+template <typename T, bool COMP>
+__global__ __launch_bounds__(256) void sourceKernel(SourceArgs s, const int64_t* __restrict__ counter,
+                                                    T* __restrict__ u, T* __restrict__ ulo) {
+    applySources<T, COMP>(s, *counter, u, ulo);
+}
+
 struct PostArgs {
-    const double* resPartial;   // [nblocks][5]
+    const double* resPartial;   // [nblocks][5] from the final stage
     int64_t nblocks;
+    double* resPartial2;        // [gridDim][5]
+    unsigned int* ticket;       // last-block ticket (returns to 0 after each step)
     double invN;                // 1/(K*Np)
     const int64_t* obsOffsets;  // [nobs+1]
     const int64_t* obsIdx;      // element-node indices
     const double* obsW;         // IDW weights
+    const int64_t* segBegin;    // [nseg][2] begin/end of observer node-list segments (<= 256 entries)
+    const int32_t* segObs;      // [nseg] observer of the segment
+    const int32_t* obsSeg0;     // [nobs+1] first segment of each observer
+    double* segPartial;         // [nseg][5]
+    int nseg;
     int nobs;
     int nf;
     double* resRing;            // [slots][5]
     double* obsRing;            // [slots][nobs][4]
     int* nonfinite;             // first slot with a non-finite residual, or -1
     int64_t* stepCounter;
+    const int64_t* batchSteps;  // number of steps in the current batch
     int computeResiduals;
 };
 
-// Residual reduction (src/solver.cpp:518-547) and inverse-distance-weighted observer
-// samples (src/solver.cpp:552-573) for the step just finished; advances the step slot.
+// End-of-step work: residuals (src/solver.cpp:518-547, deterministic two-level sum),
+// inverse-distance-weighted observers (src/solver.cpp:552-573), then the source overwrite of
+// the next step of the batch, and the step counter advance. The last block to finish the
+// first-level sums does everything after it.
 template <typename T, bool COMP>
-__global__ __launch_bounds__(256) void postKernel(PostArgs p, const T* __restrict__ u, const T* __restrict__ ulo) {
+__global__ __launch_bounds__(256) void postKernel(PostArgs p, SourceArgs s, T* __restrict__ u, T* __restrict__ ulo) {
     __shared__ double red[256][5];
+    __shared__ int amLast;
     const int tid = threadIdx.x;
     const int64_t slot = *p.stepCounter;
-    double s[5] = {0, 0, 0, 0, 0};
-    if (p.computeResiduals)
-        for (int64_t b = tid; b < p.nblocks; b += 256)
-            for (int k = 0; k < 5; ++k) s[k] += p.resPartial[b * 5 + k];
-    for (int k = 0; k < 5; ++k) red[tid][k] = s[k];
+    double acc[5] = {0, 0, 0, 0, 0};
+    if (p.computeResiduals) {
+        const int64_t chunk = (p.nblocks + gridDim.x - 1) / gridDim.x;
+        const int64_t b0 = blockIdx.x * chunk;
+        const int64_t b1 = (b0 + chunk < p.nblocks) ? b0 + chunk : p.nblocks;
+        for (int64_t b = b0 + tid; b < b1; b += 256)
+            for (int k = 0; k < 5; ++k) acc[k] += p.resPartial[b * 5 + k];
+    }
+    for (int k = 0; k < 5; ++k) red[tid][k] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w)
+            for (int k = 0; k < 5; ++k) red[tid][k] += red[tid + w][k];
+        __syncthreads();
+    }
+    if (tid < 5) p.resPartial2[blockIdx.x * 5 + tid] = red[0][tid];
+    // observer segments: one entry per thread, fixed-order tree per segment
+    for (int sg = blockIdx.x; sg < p.nseg; sg += gridDim.x) {
+        __syncthreads();
+        double ob[5] = {0, 0, 0, 0, 0};
+        const int64_t x = p.segBegin[2 * sg] + tid;
+        if (x < p.segBegin[2 * sg + 1]) {
+            const double w = p.obsW[x];
+            const int64_t base = p.obsIdx[x] * p.nf;
+            for (int k = 0; k < 4; ++k) {
+                double v = static_cast<double>(u[base + k]);
+                if constexpr (COMP) v += static_cast<double>(ulo[base + k]);
+                ob[k] = v * w;
+            }
+            ob[4] = w;
+        }
+        for (int k = 0; k < 5; ++k) red[tid][k] = ob[k];
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (tid < w)
+                for (int k = 0; k < 5; ++k) red[tid][k] += red[tid + w][k];
+            __syncthreads();
+        }
+        if (tid < 5) p.segPartial[sg * 5 + tid] = red[0][tid];
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) amLast = (atomicAdd(p.ticket, 1u) == gridDim.x - 1) ? 1 : 0;
+    __syncthreads();
+    if (!amLast) return;
+    __threadfence();
+    // second level: one first-level partial per thread, then a fixed-order tree
+    for (int k = 0; k < 5; ++k) {
+        double v = 0.0;
+        for (unsigned b = tid; b < gridDim.x; b += 256)
+            v += __hip_atomic_load(&p.resPartial2[b * 5 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        red[tid][k] = v;
+    }
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (tid < w)
@@ -456,35 +898,30 @@ __global__ __launch_bounds__(256) void postKernel(PostArgs p, const T* __restric
         __syncthreads();
     }
     if (tid < 5) {
-        double v = red[0][tid] * p.invN;
+        const double v = red[0][tid] * p.invN;
         p.resRing[slot * 5 + tid] = v;
         if (!isfinite(v)) atomicCAS(p.nonfinite, -1, static_cast<int>(slot));
     }
-    // observers: one wave per observer
-    const int wave = tid >> 6, lane = tid & 63;
-    for (int o = wave; o < p.nobs; o += 4) {
-        double acc[5] = {0, 0, 0, 0, 0};
-        for (int64_t x = p.obsOffsets[o] + lane; x < p.obsOffsets[o + 1]; x += 64) {
-            const double w = p.obsW[x];
-            const int64_t base = p.obsIdx[x] * p.nf;
-            for (int k = 0; k < 4; ++k) {
-                double v = static_cast<double>(u[base + k]);
-                if constexpr (COMP) v += static_cast<double>(ulo[base + k]);
-                acc[k] += v * w;
-            }
-            acc[4] += w;
-        }
-        for (int k = 0; k < 5; ++k)
-            for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_down(acc[k], off, 64);
-        if (lane == 0)
-            for (int k = 0; k < 4; ++k) p.obsRing[(slot * p.nobs + o) * 4 + k] = acc[k] / acc[4];
+    // observers: sum of the observer's segment partials in segment order
+    if (tid < p.nobs) {
+        double ob[5] = {0, 0, 0, 0, 0};
+        for (int sg = p.obsSeg0[tid]; sg < p.obsSeg0[tid + 1]; ++sg)
+            for (int k = 0; k < 5; ++k)
+                ob[k] += __hip_atomic_load(&p.segPartial[sg * 5 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < 4; ++k) p.obsRing[(slot * p.nobs + tid) * 4 + k] = ob[k] / ob[4];
     }
     __syncthreads();
-    if (tid == 0) *p.stepCounter = slot + 1;
+    if (slot + 1 < *p.batchSteps) applySources<T, COMP>(s, slot + 1, u, ulo);
+    __syncthreads();
+    if (tid == 0) {
+        *p.ticket = 0u;
+        *p.stepCounter = slot + 1;
+    }
 }
 
-__global__ inline void setCounterKernel(int64_t* counter, int64_t value, int* flag) {
-    *counter = value;
+__global__ void setCounterKernel(int64_t* counter, int64_t* batchSteps, int64_t nsteps, int* flag) {
+    *counter = 0;
+    *batchSteps = nsteps;
     *flag = -1;
 }
 

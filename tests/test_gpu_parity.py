@@ -4,6 +4,8 @@ FP64 bar: 1e-10 relative L2 per field (summation-order differences only); single
 evaluations 1e-12. Mixed precision bar: 1e-6 relative L2 in pressure against the GPU FP64
 path.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -27,8 +29,10 @@ RHO0 = 1.225
     (3, 2, 4, [(0x10, 1), (0x2F, 0)], (30, 0, 0), 340.0),
     (3, 1, 5, [(ALL3, 0)], (30, 0, 0), 340.0),
 ])
-def test_rhs_matches_oracle(dim, n, p, groups, v0, c0):
-    print(f"case rhs dim={dim} n={n} p={p}", flush=True)
+@pytest.mark.parametrize("kernel", ["valu", "wave"])
+def test_rhs_matches_oracle(dim, n, p, groups, v0, c0, kernel, monkeypatch):
+    print(f"case rhs dim={dim} n={n} p={p} kernel={kernel}", flush=True)
+    monkeypatch.setenv("DGFEM_KERNEL", kernel)
     m, bc, gn = make_case(dim, n, p, groups, seed=7)
     o = oracle_for(m, gn, groups, v0, RHO0, c0)
     s = dg.Solver(m.node_coords, m.element_vertices, p, bc, v0=v0, rho0=RHO0, c0=c0)
@@ -68,10 +72,12 @@ def test_rk4_2d_reflecting_matches_oracle():
     np.testing.assert_allclose(r.observers, ref["observers"], rtol=1e-8, atol=1e-12)
 
 
-def test_rk4_3d_absorbing_monopole_matches_oracle():
+@pytest.mark.parametrize("kernel", ["valu", "wave"])
+def test_rk4_3d_absorbing_monopole_matches_oracle(kernel, monkeypatch):
     """Reduced 3D case: 4^3 x 6 jittered Kuhn tets, p=3, v0=(30,0,0), c0=340, all
     absorbing, monopole at the origin."""
-    print("case rk4 3d absorbing monopole n4 p3", flush=True)
+    print(f"case rk4 3d absorbing monopole n4 p3 kernel={kernel}", flush=True)
+    monkeypatch.setenv("DGFEM_KERNEL", kernel)
     groups = [(ALL3, 0)]
     v0 = (30.0, 0.0, 0.0)
     m, bc, gn = make_case(3, 4, 3, groups, jitter=0.15, seed=2)
@@ -127,10 +133,12 @@ def test_graph_replay_equals_eager():
     assert np.array_equal(outs[0][1], outs[1][1])
 
 
-def test_mixed_precision_close_to_fp64():
+@pytest.mark.parametrize("kernel", ["valu", "wave"])
+def test_mixed_precision_close_to_fp64(kernel, monkeypatch):
     """Mixed precision (FP32 state / fluxes, compensated RK4 accumulation) against GPU FP64,
     3D absorbing case with a monopole, 500 steps: pressure rel L2 <= 1e-6."""
-    print("case mixed vs fp64 3d", flush=True)
+    print(f"case mixed vs fp64 3d kernel={kernel}", flush=True)
+    monkeypatch.setenv("DGFEM_KERNEL", kernel)
     groups = [(ALL3, 0)]
     m, bc, gn = make_case(3, 6, 3, groups, jitter=0.15, seed=2)
     res = []
