@@ -34,6 +34,18 @@ HBM_PEAK_GBPS = 8000.0
 BYTES_PER_DOF = {"fp64": 212.0, "mixed": 115.0}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH/WRITE summary."""
+    path = os.path.join(ROOT, "profiles", "stage_kernel_c2_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch " + kernel + ":"):
+                return float(line.split(":")[-1])
+    except OSError:
+        pass
+    return None
+
+
 def build_case(n, seed=2):
     import dgfem_amd as dg
 
@@ -140,6 +152,8 @@ def main():
     rel_o = float(np.linalg.norm(omx[..., 0] - o64[..., 0]) / max(np.linalg.norm(o64[..., 0]), 1e-300))
 
     mixed = out["mixed"]
+    kernel_name = ("stageKernelWave<3,3,float,true>" if os.environ.get("DGFEM_KERNEL") == "wave"
+                   else "stageKernel<3,3,4,float,true>")
     stage_sum_ms = sum(mixed["stage_ms"])
     achieved = BYTES_PER_DOF["mixed"] * dof / (stage_sum_ms * 1e-3) / 1e9
     stage_sum64 = sum(out["fp64"]["stage_ms"])
@@ -174,8 +188,9 @@ def main():
         "rel_l2_pressure_vs_fp64": rel_p,
         "rel_l2_observer_p_vs_fp64": rel_o,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": "stageKernel<3,3,4,float,true>",
+                     "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": measured_traffic(kernel_name) if args.n == 32 else None,
+                     "kernel": kernel_name,
                      "stage_ms": mixed["stage_ms"],
                      "bytes_per_dof_update": BYTES_PER_DOF["mixed"]},
         "cpu_baseline": cpu,

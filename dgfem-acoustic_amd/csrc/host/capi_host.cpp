@@ -62,6 +62,17 @@ dg_status dg_ref_operators(int32_t dim, int32_t order, double* Dw, double* LIFT,
     return DG_OK;
 }
 
+dg_status dg_ref_mass(int32_t dim, int32_t order, double* M) {
+    if (!M) return DG_ERR_INVALID;
+    try {
+        const dg::RefElement& R = dg::refElement(dim, order);
+        std::memcpy(M, R.M.data(), R.M.size() * sizeof(double));
+    } catch (const std::exception&) {
+        return DG_ERR_UNSUPPORTED;
+    }
+    return DG_OK;
+}
+
 dg_status dg_mesh_precompute(const dg_mesh_desc* m, double* Jinv, int32_t* nbr, uint8_t* code, int32_t* face_perm,
                              int32_t* to_canonical) {
     if (!m || !m->node_coords || !m->element_vertices) return DG_ERR_INVALID;

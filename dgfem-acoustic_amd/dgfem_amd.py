@@ -104,6 +104,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                                             P(C.c_uint32), P(C.c_int8), P(C.c_int8)]),
         "dg_ref_operators": (C.c_int32, [C.c_int32, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_int32),
                                          P(C.c_double), P(C.c_int32)]),
+        "dg_ref_mass": (C.c_int32, [C.c_int32, C.c_int32, P(C.c_double)]),
         "dg_mesh_precompute": (C.c_int32, [P(MeshDesc), P(C.c_double), P(C.c_int32), P(C.c_uint8), P(C.c_int32),
                                            P(C.c_int32)]),
     }
@@ -188,7 +189,11 @@ def ref_operators(dim: int, p: int) -> dict:
                               _ptr(Nr, C.c_double), _ptr(lat, C.c_int32))
     if st != DG_OK:
         raise DGError(st, "dg_ref_operators failed")
-    return dict(Dw=Dw, LIFT=L, fmask=fm, Nref=Nr, lattice=lat, Np=Np, Nfp=Nfp)
+    M = np.zeros((Np, Np))
+    st = lib.dg_ref_mass(dim, p, _ptr(M, C.c_double))
+    if st != DG_OK:
+        raise DGError(st, "dg_ref_mass failed")
+    return dict(Dw=Dw, LIFT=L, fmask=fm, Nref=Nr, lattice=lat, Np=Np, Nfp=Nfp, M=M)
 
 
 def mesh_precompute(node_coords, element_vertices, order, face_bc=None) -> dict:

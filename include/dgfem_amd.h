@@ -167,6 +167,8 @@ DG_API dg_status dg_resolve_bc_sides(int32_t dim, int64_t num_elements, const in
      Nref [(dim+1)][3], lattice [Np][4] integer barycentrics. Any pointer may be NULL. */
 DG_API dg_status dg_ref_operators(int32_t dim, int32_t order, double* Dw, double* LIFT, int32_t* fmask, double* Nref,
                                   int32_t* lattice);
+/* Reference mass matrix M [Np][Np] of (dim, p) on Gmsh's reference simplex, canonical order. */
+DG_API dg_status dg_ref_mass(int32_t dim, int32_t order, double* M);
 /* Mesh precompute (src/Mesh.cpp:20-686 counterpart): Jinv [K][3][3], neighbour [K][dim+1]
    (-1-bc on the boundary), neighbour code [K][dim+1] (local face | perm << 2),
    face_perm [nperm][Nfp], to_canonical [K][Np]. Any output pointer may be NULL. */

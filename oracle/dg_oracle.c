@@ -240,6 +240,47 @@ static void normalize3(double* v) {
     if (n > 0) { v[0] /= n; v[1] /= n; v[2] /= n; }
 }
 
+/* ------------------------------------------------------------ boundary fluxes ---- */
+/* R*K*R^-1 of the absorbing condition at one point (src/Mesh.cpp:628-663). */
+static void rkr_matrix(const oracle_physics* ph, const double* n, const double* t, const double* s, double* R) {
+    double c0 = ph->c0, rho0 = ph->rho0;
+    double vn0 = ph->v0[0] * n[0] + ph->v0[1] * n[1] + ph->v0[2] * n[2];
+    double lambda = (vn0 < 0) ? 0 : -1;
+    R[0] = 0.25 * (c0 + vn0);
+    for (int q = 0; q < 3; ++q) R[1 + q] = 0.25 * (c0 * rho0 * (c0 + vn0) * n[q]);
+    for (int r = 0; r < 3; ++r) {
+        R[(r + 1) * 4] = 0.25 * (n[r] * (c0 + vn0) / (rho0 * c0));
+        for (int q = 0; q < 3; ++q)
+            R[(r + 1) * 4 + 1 + q] = 0.25 * ((c0 + vn0) * n[r] * n[q] - vn0 * lambda * (t[q] * t[r] + s[q] * s[r]));
+    }
+}
+
+/* Reflecting (rigid wall) flux n . F(p, v - (v.n)n) at one point (src/Mesh.cpp:911-949). */
+static void reflect_flux(const oracle_physics* ph, const double* n, const double* q, double* out) {
+    const double* v0 = ph->v0;
+    double rho0 = ph->rho0, c0 = ph->c0;
+    double ug[4] = {q[0], q[1], q[2], q[3]};
+    double dot = n[0] * ug[1] + n[1] * ug[2] + n[2] * ug[3];
+    ug[1] -= dot * n[0]; ug[2] -= dot * n[1]; ug[3] -= dot * n[2];
+    double FG[4][3] = {
+        {v0[0] * ug[0] + rho0 * c0 * c0 * ug[1], v0[1] * ug[0] + rho0 * c0 * c0 * ug[2], v0[2] * ug[0] + rho0 * c0 * c0 * ug[3]},
+        {v0[0] * ug[1] + ug[0] / rho0, v0[1] * ug[1], v0[2] * ug[1]},
+        {v0[0] * ug[2], v0[1] * ug[2] + ug[0] / rho0, v0[2] * ug[2]},
+        {v0[0] * ug[3], v0[1] * ug[3], v0[2] * ug[3] + ug[0] / rho0}};
+    for (int k = 0; k < 4; ++k) out[k] = n[0] * FG[k][0] + n[1] * FG[k][1] + n[2] * FG[k][2];
+}
+
+void oracle_bc_flux(const oracle_physics* ph, int bc, const double n[3], const double t[3], const double s[3],
+                    const double q[4], double out[4]) {
+    if (bc == 1) {
+        reflect_flux(ph, n, q, out);
+    } else {
+        double R[16];
+        rkr_matrix(ph, n, t, s, R);
+        for (int k = 0; k < 4; ++k) out[k] = R[k * 4] * q[0] + R[k * 4 + 1] * q[1] + R[k * 4 + 2] * q[2] + R[k * 4 + 3] * q[3];
+    }
+}
+
 /* -------------------------------------------------------------- construction ---- */
 /* Mesh precompute: src/Mesh.cpp:20-686 (geometry, faces, orientation, BCs, RKR) and
    precomputeMassMatrix / getElMassMatrix (src/Mesh.cpp:691-724). */
@@ -498,20 +539,7 @@ oracle_ctx* oracle_create(const oracle_mesh* m, const oracle_physics* ph) {
     /* RKR absorbing matrices (src/Mesh.cpp:619-666) */
     for (int64_t f = 0; f < F; ++f) {
         if (!c->fBnd[f] || c->fBC[f] != 0) continue;
-        const double* n = c->fN + f * 3;
-        const double* t = c->fT + f * 3;
-        const double* s = c->fB + f * 3;
-        double c0 = ph->c0, rho0 = ph->rho0;
-        double vn0 = ph->v0[0] * n[0] + ph->v0[1] * n[1] + ph->v0[2] * n[2];
-        double lambda = (vn0 < 0) ? 0 : -1;
-        double* R = c->RKR + f * 16;
-        R[0] = 0.25 * (c0 + vn0);
-        for (int q = 0; q < 3; ++q) R[1 + q] = 0.25 * (c0 * rho0 * (c0 + vn0) * n[q]);
-        for (int r = 0; r < 3; ++r) {
-            R[(r + 1) * 4] = 0.25 * (n[r] * (c0 + vn0) / (rho0 * c0));
-            for (int q = 0; q < 3; ++q)
-                R[(r + 1) * 4 + 1 + q] = 0.25 * ((c0 + vn0) * n[r] * n[q] - vn0 * lambda * (t[q] * t[r] + s[q] * s[r]));
-        }
+        rkr_matrix(ph, c->fN + f * 3, c->fT + f * 3, c->fB + f * 3, c->RKR + f * 16);
     }
 
     c->Flux = calloc((size_t)4 * K * Np * 3, sizeof(double));
@@ -569,14 +597,7 @@ static void update_flux(oracle_ctx* c, const double* u) {
             }
             double out[4];
             if (c->fBC[f] == 1) {
-                double dot = n[0] * ug[1] + n[1] * ug[2] + n[2] * ug[3];
-                ug[1] -= dot * n[0]; ug[2] -= dot * n[1]; ug[3] -= dot * n[2];
-                double FG[4][3] = {
-                    {v0[0] * ug[0] + rho0 * c0 * c0 * ug[1], v0[1] * ug[0] + rho0 * c0 * c0 * ug[2], v0[2] * ug[0] + rho0 * c0 * c0 * ug[3]},
-                    {v0[0] * ug[1] + ug[0] / rho0, v0[1] * ug[1], v0[2] * ug[1]},
-                    {v0[0] * ug[2], v0[1] * ug[2] + ug[0] / rho0, v0[2] * ug[2]},
-                    {v0[0] * ug[3], v0[1] * ug[3], v0[2] * ug[3] + ug[0] / rho0}};
-                for (int q = 0; q < 4; ++q) out[q] = n[0] * FG[q][0] + n[1] * FG[q][1] + n[2] * FG[q][2];
+                reflect_flux(&c->ph, n, ug, out);
             } else {
                 const double* R = c->RKR + f * 16;
                 for (int q = 0; q < 4; ++q)

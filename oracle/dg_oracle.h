@@ -94,6 +94,12 @@ int oracle_rhs(oracle_ctx* ctx, const double* u, double* k, double dt, int num_t
 /* The time loop of rungeKutta / forwardEuler on state u ([4][K*Np], updated in place). */
 int oracle_run(oracle_ctx* ctx, double* u, const oracle_run_params* prm, oracle_run_output* out);
 
+/* Boundary flux at one face point (Mesh::updateFlux, src/Mesh.cpp:911-969): bc 1 reflecting
+   (n . F of the trace with its normal velocity removed), bc 0 absorbing (RKR q with the
+   matrix of src/Mesh.cpp:619-666 built from n, tangent t and bitangent s). */
+void oracle_bc_flux(const oracle_physics* ph, int bc, const double n[3], const double t[3], const double s[3],
+                    const double q[4], double out[4]);
+
 /* Quadrature sizes used (for reports): volume and face points. */
 int oracle_num_volume_points(const oracle_ctx* ctx);
 int oracle_num_face_points(const oracle_ctx* ctx);

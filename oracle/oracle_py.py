@@ -72,6 +72,7 @@ def load(build_if_missing: bool = True) -> C.CDLL:
     lib.oracle_num_face_points.argtypes = [C.c_void_p]
     lib.oracle_num_faces.argtypes = [C.c_void_p]
     lib.oracle_num_faces.restype = C.c_int64
+    lib.oracle_bc_flux.argtypes = [P(OPhysics), C.c_int] + [P(C.c_double)] * 5
     _lib = lib
     return lib
 
@@ -92,6 +93,16 @@ def group_node_sets(element_vertices: np.ndarray, face_side: np.ndarray, groups)
             sel = (side >= 0) & ((np.uint32(mask) >> np.clip(side, 0, 31).astype(np.uint32)) & 1).astype(bool)
             verts.append(element_vertices[sel][:, fv[f]].ravel())
         out.append(np.unique(np.concatenate(verts)) if verts else np.zeros(0, np.int64))
+    return out
+
+
+def bc_flux(bc, n, t, s, q, v0=(0, 0, 0), rho0=1.0, c0=1.0):
+    """Boundary flux of the oracle at one face point (bc 0 absorbing, 1 reflecting)."""
+    lib = load()
+    ph = OPhysics((C.c_double * 3)(*v0), rho0, c0)
+    arr = [np.ascontiguousarray(a, dtype=np.float64) for a in (n, t, s, q)]
+    out = np.zeros(4)
+    lib.oracle_bc_flux(C.byref(ph), bc, *[_p(a, C.c_double) for a in arr], _p(out, C.c_double))
     return out
 
 
