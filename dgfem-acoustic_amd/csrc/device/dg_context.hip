@@ -891,7 +891,8 @@ dg_status dg_run(dg_context* c, const dg_run_params* prm, dg_step_fn on_step, dg
             if (flag >= 0 && resid) lim = flag + 1;
             for (int64_t s = 0; s < lim; ++s) {
                 if (on_step) {
-                    dg_step_record rec{done + s, times[done + s], &c->hRes[s * 5], no, no ? &c->hObs[s * no * 4] : nullptr};
+                    dg_step_record rec{done + s, times[done + s], &c->hRes[s * 5], no, no ? &c->hObs[s * no * 4] : nullptr,
+                                       1e-3 * ms / static_cast<double>(n)};
                     if (on_step(user, &rec) != 0) {
                         (void)hipEventDestroy(ev0);
                         (void)hipEventDestroy(ev1);

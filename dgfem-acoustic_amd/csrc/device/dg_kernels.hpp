@@ -462,7 +462,7 @@ struct WaveShape {
 };
 
 template <int DIM, int P, typename T, bool COMP>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void stageKernelWave(
+__global__ __launch_bounds__(kBlock) void stageKernelWave(
     StageArgs<T> a, const T* __restrict__ afrag) {
     using S = Sizes<DIM, P>;
     using L = OpLayout<DIM, P>;

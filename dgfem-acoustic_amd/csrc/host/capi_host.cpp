@@ -94,3 +94,95 @@ dg_status dg_mesh_precompute(const dg_mesh_desc* m, double* Jinv, int32_t* nbr, 
 }
 
 }  // extern "C"
+
+// ---- front-end helpers ----
+#include <sstream>
+
+#include "config.hpp"
+#include "formula.hpp"
+#include "sources_io.hpp"
+
+extern "C" {
+
+double dg_eval_formula(const char* expression, double t) {
+    return expression ? dg::evalFormula(expression, t) : -1.0;
+}
+
+dg_status dg_read_source_table(const char* path, double* rows, int64_t capacity, int64_t* num_rows) {
+    if (!path) return DG_ERR_INVALID;
+    try {
+        const std::string p(path);
+        const std::string ext = p.substr(p.find_last_of('.') + 1);
+        std::vector<std::vector<double>> t =
+            ext == "wav" ? dg::parseWavTable(p) : dg::parseCsvTable(p, ';');
+        if (num_rows) *num_rows = static_cast<int64_t>(t.size());
+        for (int64_t i = 0; rows && i < capacity && i < static_cast<int64_t>(t.size()); ++i) {
+            rows[2 * i] = t[i].size() > 0 ? t[i][0] : 0.0;
+            rows[2 * i + 1] = t[i].size() > 1 ? t[i][1] : 0.0;
+        }
+    } catch (const std::exception&) {
+        return DG_ERR_INVALID;
+    }
+    return DG_OK;
+}
+
+dg_status dg_case_summary(const char* config_path, char* out, int64_t capacity) {
+    if (!config_path || !out || capacity < 2) return DG_ERR_INVALID;
+    std::ostringstream js;
+    dg_status st = DG_OK;
+    try {
+        dg::Case cs = dg::loadCase(config_path);
+        const dg::Config& c = cs.config;
+        std::vector<int8_t> bc = dg::boundaryTypes(cs);
+        std::vector<int64_t> verts(cs.mesh.numElements() * (cs.mesh.dim + 1));
+        for (int64_t e = 0; e < cs.mesh.numElements(); ++e)
+            for (int v = 0; v <= cs.mesh.dim; ++v)
+                verts[e * (cs.mesh.dim + 1) + v] = cs.mesh.elementNodes[e * cs.mesh.nodesPerElement + v];
+        std::vector<uint8_t> isB = dg::boundaryFaceFlags(cs.mesh.dim, cs.mesh.numElements(), verts.data());
+        int64_t nAbs = 0, nRef = 0;
+        for (size_t i = 0; i < isB.size(); ++i)
+            if (isB[i]) (bc[i] == 1 ? nRef : nAbs)++;
+        js.precision(17);
+        js << "{\"mesh\":\"" << c.meshFileName << "\",\"dim\":" << cs.mesh.dim << ",\"order\":" << cs.mesh.order
+           << ",\"elements\":" << cs.mesh.numElements() << ",\"nodes_per_element\":" << cs.mesh.nodesPerElement
+           << ",\"time\":[" << c.timeStart << "," << c.timeEnd << "," << c.timeStep << "," << c.timeRate << "]"
+           << ",\"integrator\":\"" << c.timeIntMethod << "\",\"num_threads\":" << c.numThreads << ",\"mean_flow\":["
+           << c.v0[0] << "," << c.v0[1] << "," << c.v0[2] << "," << c.rho0 << "," << c.c0 << "]"
+           << ",\"bc_counts\":{\"absorbing\":" << nAbs << ",\"reflecting\":" << nRef << "},\"physical_bcs\":[";
+        bool first = true;
+        for (const auto& kv : c.physBCs) {
+            js << (first ? "" : ",") << "[" << kv.first << ",\"" << kv.second.first << "\"]";
+            first = false;
+        }
+        js << "],\"sources\":[";
+        for (size_t i = 0; i < c.sources.size(); ++i) {
+            const dg::SourceDef& s = c.sources[i];
+            js << (i ? "," : "") << "{\"kind\":" << static_cast<int>(s.kind) << ",\"pole\":" << s.pole << ",\"x\":" << s.x
+               << ",\"y\":" << s.y << ",\"z\":" << s.z << ",\"size\":" << s.size << ",\"amp\":" << s.amp
+               << ",\"freq\":" << s.freq << ",\"phase\":" << s.phase << ",\"duration\":" << s.duration
+               << ",\"formula\":\"" << s.formula << "\",\"rows\":" << s.table.size() << "}";
+        }
+        js << "],\"observers\":[";
+        for (size_t i = 0; i < c.observers.size(); ++i)
+            js << (i ? "," : "") << "[" << c.observers[i][0] << "," << c.observers[i][1] << "," << c.observers[i][2] << ","
+               << c.observers[i][3] << "]";
+        js << "],\"initial_conditions\":[";
+        for (size_t i = 0; i < c.initConditions.size(); ++i) {
+            js << (i ? "," : "") << "[";
+            for (int k = 0; k < 6; ++k) js << (k ? "," : "") << c.initConditions[i][k];
+            js << "]";
+        }
+        js << "]}";
+    } catch (const std::exception& ex) {
+        js.str("");
+        js << ex.what();
+        st = DG_ERR_INVALID;
+    }
+    const std::string s = js.str();
+    const size_t n = std::min<size_t>(s.size(), static_cast<size_t>(capacity - 1));
+    std::memcpy(out, s.data(), n);
+    out[n] = 0;
+    return st;
+}
+
+}  // extern "C"

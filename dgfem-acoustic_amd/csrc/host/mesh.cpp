@@ -218,6 +218,27 @@ std::vector<int8_t> resolveBoundaryTypesFromNodeSets(int dim, int64_t K, const i
     return bc;
 }
 
+std::vector<uint8_t> boundaryFaceFlags(int dim, int64_t K, const int64_t* elemVerts) {
+    const int nvpe = dim + 1;
+    std::vector<std::pair<std::array<int64_t, 3>, int64_t>> keys(K * nvpe);
+    for (int64_t e = 0; e < K; ++e)
+        for (int f = 0; f < nvpe; ++f) {
+            std::array<int64_t, 3> k = {-1, -1, -1};
+            for (int v = 0; v < dim; ++v) k[v] = elemVerts[e * nvpe + ((dim == 2) ? kFaceVerts2D[f][v] : kFaceVerts3D[f][v])];
+            std::sort(k.begin(), k.begin() + dim);
+            keys[e * nvpe + f] = {k, e * nvpe + f};
+        }
+    std::sort(keys.begin(), keys.end());
+    std::vector<uint8_t> flags(K * nvpe, 0);
+    for (size_t i = 0; i < keys.size();) {
+        size_t j = i + 1;
+        while (j < keys.size() && keys[j].first == keys[i].first) ++j;
+        if (j - i == 1) flags[keys[i].second] = 1;
+        i = j;
+    }
+    return flags;
+}
+
 DGMesh buildDGMesh(int dim, int order, int64_t K, const double* nodeCoords, const int64_t* elemVerts,
                    const int8_t* faceBC) {
     if (dim != 2 && dim != 3) throw std::runtime_error("dimension must be 2 or 3");
@@ -240,7 +261,6 @@ DGMesh buildDGMesh(int dim, int order, int64_t K, const double* nodeCoords, cons
     m.detJ.assign(K, 0.0);
     m.inradius.assign(K, 0.0);
     std::string err;
-#pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < K; ++e) {
         const double* X = nodeCoords + e * Np * 3;
         double J[3][3] = {{0}};
@@ -267,7 +287,6 @@ DGMesh buildDGMesh(int dim, int order, int64_t K, const double* nodeCoords, cons
             Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
         }
         if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) {
-#pragma omp critical
             err = "degenerate element " + std::to_string(e);
             continue;
         }
@@ -296,7 +315,6 @@ DGMesh buildDGMesh(int dim, int order, int64_t K, const double* nodeCoords, cons
             }
             auto it = canon.find(baryKey(b, p));
             if (!ok || sum != p || it == canon.end()) {
-#pragma omp critical
                 err = "element " + std::to_string(e) + " node " + std::to_string(a) +
                       " is not on the straight-sided equispaced lattice";
                 break;
@@ -333,7 +351,6 @@ DGMesh buildDGMesh(int dim, int order, int64_t K, const double* nodeCoords, cons
         int64_t ef;
     };
     std::vector<FK> keys(K * nvpe);
-#pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < K; ++e)
         for (int f = 0; f < nvpe; ++f) {
             std::array<int64_t, 3> k = {-1, -1, -1};

@@ -64,6 +64,9 @@ std::vector<int8_t> resolveBoundaryTypesFromNodeSets(int dim, int64_t K, const i
                                                      const std::vector<std::vector<int64_t>>& groupNodes,
                                                      const std::vector<int8_t>& groupType);
 
+// 1 for every (element, local face) that no other element shares (src/Mesh.cpp:536-568).
+std::vector<uint8_t> boundaryFaceFlags(int dim, int64_t K, const int64_t* elemVerts);
+
 // Device-ready FP64 precompute in canonical node order.
 struct DGMesh {
     int dim = 0, p = 0, Np = 0, Nfp = 0, Nfaces = 0;

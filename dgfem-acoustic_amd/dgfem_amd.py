@@ -52,7 +52,7 @@ class RunParams(C.Structure):
 
 class StepRecord(C.Structure):
     _fields_ = [("step", C.c_int64), ("t", C.c_double), ("residuals", C.POINTER(C.c_double)),
-                ("num_observers", C.c_int32), ("observers", C.POINTER(C.c_double))]
+                ("num_observers", C.c_int32), ("observers", C.POINTER(C.c_double)), ("step_seconds", C.c_double)]
 
 
 class Info(C.Structure):
@@ -104,6 +104,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                                             P(C.c_uint32), P(C.c_int8), P(C.c_int8)]),
         "dg_ref_operators": (C.c_int32, [C.c_int32, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_int32),
                                          P(C.c_double), P(C.c_int32)]),
+        "dg_eval_formula": (C.c_double, [C.c_char_p, C.c_double]),
+        "dg_read_source_table": (C.c_int32, [C.c_char_p, P(C.c_double), C.c_int64, P(C.c_int64)]),
+        "dg_case_summary": (C.c_int32, [C.c_char_p, C.c_char_p, C.c_int64]),
         "dg_ref_mass": (C.c_int32, [C.c_int32, C.c_int32, P(C.c_double)]),
         "dg_mesh_precompute": (C.c_int32, [P(MeshDesc), P(C.c_double), P(C.c_int32), P(C.c_uint8), P(C.c_int32),
                                            P(C.c_int32)]),
@@ -169,6 +172,31 @@ def resolve_bc_sides(mesh: BoxMesh, groups: list[tuple[int, int]]) -> np.ndarray
     if st != DG_OK:
         raise DGError(st, "dg_resolve_bc_sides failed")
     return out
+
+
+def eval_formula(expr: str, t: float) -> float:
+    return load_library().dg_eval_formula(expr.encode(), t)
+
+
+def read_source_table(path: str) -> np.ndarray:
+    lib = load_library()
+    n = C.c_int64()
+    st = lib.dg_read_source_table(path.encode(), None, 0, C.byref(n))
+    if st != DG_OK:
+        raise DGError(st, f"cannot read source table {path}")
+    rows = np.zeros((n.value, 2))
+    lib.dg_read_source_table(path.encode(), _ptr(rows, C.c_double), n.value, C.byref(n))
+    return rows
+
+
+def case_summary(config_path: str) -> dict:
+    import json
+    lib = load_library()
+    buf = C.create_string_buffer(1 << 20)
+    st = lib.dg_case_summary(config_path.encode(), buf, len(buf))
+    if st != DG_OK:
+        raise DGError(st, buf.value.decode())
+    return json.loads(buf.value.decode())
 
 
 def num_nodes(dim: int, p: int) -> int:

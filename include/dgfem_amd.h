@@ -110,6 +110,7 @@ typedef struct {
     const double* residuals; /* [5]: p, rho, vx, vy, vz */
     int32_t num_observers;
     const double* observers; /* [num_observers][4]: p, vx, vy, vz */
+    double step_seconds;     /* device time of the step's batch divided by its steps */
 } dg_step_record;
 
 typedef int (*dg_step_fn)(void* user, const dg_step_record* rec);
@@ -160,6 +161,22 @@ DG_API dg_status dg_box_mesh_fill(int32_t dim, int32_t n, int32_t order, double 
 DG_API dg_status dg_resolve_bc_sides(int32_t dim, int64_t num_elements, const int64_t* element_vertices,
                                      const int8_t* face_side, int32_t num_groups, const uint32_t* group_side_mask,
                                      const int8_t* group_type, int8_t* face_bc);
+
+/* ---- front-end helpers (no device needed) ---- */
+
+/* Value of a `formula` source expression at time t with the reference's evaluator semantics
+   (src/eqEdit.cpp:107-311); -1 for malformed expressions, as the reference. */
+DG_API double dg_eval_formula(const char* expression, double t);
+
+/* (time, value) table of a `file` source (.csv: src/utils.cpp:138-177; .wav: src/utils.cpp:
+   240-258). Writes up to `capacity` rows into rows[capacity][2]; *num_rows = total rows. */
+DG_API dg_status dg_read_source_table(const char* path, double* rows, int64_t capacity, int64_t* num_rows);
+
+/* Parse a .conf/.json configuration and its mesh (src/configParser.cpp:33-450), resolve the
+   boundary types, and write a JSON summary into out[capacity] (keys: mesh, dim, order,
+   elements, nodes_per_element, time, integrator, mean_flow, bc_counts, sources, observers,
+   initial_conditions, physical_bcs). */
+DG_API dg_status dg_case_summary(const char* config_path, char* out, int64_t capacity);
 
 /* Diagnostics of the host precompute (no device needed), for tests of the host logic.
    Reference operators of (dim, p) in the canonical lattice order:
