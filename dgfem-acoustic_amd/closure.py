@@ -1,0 +1,322 @@
+"""Learned absorbing-boundary closure: training data from FP64 runs on an enlarged domain,
+training with the library's GPU kernels, and evaluation against the characteristic
+absorbing BC of the reference (src/Mesh.cpp:619-666, :954-969) on the truncated domain.
+
+The truncated mesh is the part of the enlarged mesh inside [-1,1]^2, element for element
+(vertices on the lines x = +-1, y = +-1 are kept on them), so that a truncated run differs
+from the enlarged one only by what happens on the truncation faces. The network, its
+training kernels and the in-solver inference live in libdgfem_amd.so (dg_closure_*,
+dg_set_closure); this module drives them.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import dgfem_amd as dg
+
+
+@dataclass
+class ClosureCase:
+    n_trunc: int = 100          # cells per side of [-1,1]^2 (h = 2 / n_trunc)
+    factor: int = 3             # enlarged domain [-factor, factor]^2
+    order: int = 4
+    jitter: float = 0.2
+    seed: int = 4
+    c0: float = 340.0
+    rho0: float = 1.225
+    source_size: float = 0.05   # monopole node radius
+    n_observers: int = 8        # on the circle r = observer_radius
+    observer_radius: float = 0.8
+    observer_size: float = 0.03
+    cfl: float = 1.0            # dt = cfl * r_min / ((c0 + |v0|max) (2p + 1))
+    vmax: float = 0.3           # |v0| <= vmax * c0
+    travel: float = 3.0         # simulated time = travel / c0 (source -> boundary -> observers)
+    steps: int = 0              # 0: from travel and dt
+
+
+@dataclass
+class Meshes:
+    parent_coords: np.ndarray
+    parent_verts: np.ndarray
+    sub_coords: np.ndarray
+    sub_verts: np.ndarray
+    sub_to_parent: np.ndarray
+    faces_sub: tuple            # (element, local face) of the truncation faces in the truncated mesh
+    faces_parent: tuple         # the same faces in the enlarged mesh
+    r_min: float = 0.0
+
+
+def _snap(mesh: dg.BoxMesh, lo: float, h: float, lines=(-1.0, 1.0)) -> np.ndarray:
+    """Node coordinates with every vertex whose grid line is x = +-1 or y = +-1 put back on
+    that line; the other nodes are rebuilt from the element's (moved) vertices."""
+    X = mesh.node_coords
+    K, Np, _ = X.shape
+    V = X[:, :3, :2]
+    # barycentric coordinates of the element nodes (same local pattern for every element)
+    A = np.stack([V[0, 1] - V[0, 0], V[0, 2] - V[0, 0]], axis=1)
+    xi = np.linalg.solve(A, (X[0, :, :2] - V[0, 0]).T).T
+    lam = np.stack([1 - xi[:, 0] - xi[:, 1], xi[:, 0], xi[:, 1]], axis=1)
+    rebuilt = np.einsum("ak,ekc->eac", lam, V)
+    assert np.abs(rebuilt - X[:, :, :2]).max() < 1e-10, "generator node pattern is not affine-uniform"
+    Vs = V.copy()
+    for c in range(2):
+        g = np.rint((Vs[..., c] - lo) / h)
+        for ln in lines:
+            on = np.abs(g * h + lo - ln) < 1e-9
+            Vs[..., c][on] = ln
+    out = X.copy()
+    out[:, :, :2] = np.einsum("ak,ekc->eac", lam, Vs)
+    return out
+
+
+def _cross2(a, b):
+    return a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]
+
+
+def build_meshes(case: ClosureCase) -> Meshes:
+    F = case.factor
+    n = case.n_trunc * F
+    h = 2.0 / case.n_trunc
+    m = dg.box_mesh(2, n, case.order, jitter=case.jitter, seed=case.seed, lo=-float(F), hi=float(F))
+    coords = _snap(m, -float(F), h)
+    cen = coords[:, :3, :2].mean(axis=1)
+    inside = np.where((np.abs(cen[:, 0]) < 1) & (np.abs(cen[:, 1]) < 1))[0]
+    sub_coords = np.ascontiguousarray(coords[inside])
+    sub_verts = np.ascontiguousarray(m.element_vertices[inside])
+    pre = dg.mesh_precompute(sub_coords, sub_verts, case.order)
+    e, f = np.nonzero(pre["nbr"] < 0)
+    # inradius of the sub-mesh elements (CFL)
+    V = sub_coords[:, :3, :2]
+    a = np.linalg.norm(V[:, 1] - V[:, 2], axis=1)
+    b = np.linalg.norm(V[:, 0] - V[:, 2], axis=1)
+    c = np.linalg.norm(V[:, 0] - V[:, 1], axis=1)
+    area = 0.5 * np.abs(_cross2(V[:, 1] - V[:, 0], V[:, 2] - V[:, 0]))
+    r_in = 2 * area / (a + b + c)
+    Vp = coords[:, :3, :2]
+    ap = np.linalg.norm(Vp[:, 1] - Vp[:, 2], axis=1) + np.linalg.norm(Vp[:, 0] - Vp[:, 2], axis=1) + \
+        np.linalg.norm(Vp[:, 0] - Vp[:, 1], axis=1)
+    areap = 0.5 * np.abs(_cross2(Vp[:, 1] - Vp[:, 0], Vp[:, 2] - Vp[:, 0]))
+    r_min = float(min(r_in.min(), (2 * areap / ap).min()))
+    return Meshes(coords, m.element_vertices, sub_coords, sub_verts, inside, (e.astype(np.int64), f.astype(np.int32)),
+                  (inside[e].astype(np.int64), f.astype(np.int32)), r_min)
+
+
+def observers(case: ClosureCase):
+    ang = 2 * np.pi * (np.arange(case.n_observers) + 0.5) / case.n_observers
+    r = case.observer_radius
+    return [(r * np.cos(a), r * np.sin(a), 0.0, case.observer_size) for a in ang]
+
+
+def time_step(case: ClosureCase, meshes: Meshes) -> tuple[float, int]:
+    dt = case.cfl * meshes.r_min / ((1 + case.vmax) * case.c0 * (2 * case.order + 1))
+    steps = case.steps or int(np.ceil(case.travel / case.c0 / dt))
+    return dt, steps
+
+
+def scenario(case: ClosureCase, seed: int) -> dict:
+    """A monopole in the disk r < 0.5 at 20-200 Hz, or a Gaussian pulse of width 0.05-0.2,
+    with a mean flow of |v0| <= vmax c0 in a random direction."""
+    rng = np.random.default_rng(seed)
+    r, a = 0.5 * np.sqrt(rng.uniform()), rng.uniform(0, 2 * np.pi)
+    va, vm = rng.uniform(0, 2 * np.pi), case.vmax * case.c0 * np.sqrt(rng.uniform())
+    s = dict(x=r * np.cos(a), y=r * np.sin(a), v0=(vm * np.cos(va), vm * np.sin(va), 0.0))
+    if rng.uniform() < 0.5:
+        s.update(kind="monopole", freq=rng.uniform(20, 200), amp=1.0)
+    else:
+        s.update(kind="gaussian", width=rng.uniform(0.05, 0.2), amp=1.0)
+    return s
+
+
+def _solver(coords, verts, case, scen, face_bc=None, precision=dg.DG_FP64):
+    s = dg.Solver(coords, verts, case.order, face_bc, v0=scen["v0"], rho0=case.rho0, c0=case.c0, precision=precision,
+                  steps_per_graph=16)
+    if scen["kind"] == "monopole":
+        s.add_monopole(scen["x"], scen["y"], 0.0, case.source_size, scen["amp"], scen["freq"], 0.0, 1e9)
+    else:
+        s.add_gaussian(scen["x"], scen["y"], 0.0, scen["width"] ** 2, scen["amp"])
+    for o in observers(case):
+        s.add_observer(*o)
+    return s
+
+
+def enlarged_run(case, meshes, scen, dt, steps, record_every=1, face_fraction=1.0, seed=0, min_scale=0.0,
+                 target_clip=10.0):
+    """FP64 run on the enlarged mesh. Returns the observer pressure series [steps][nobs] and
+    training samples (X, Y) from the truncation-face traces every record_every steps (faces
+    whose trace scale is below min_scale times the run's largest are dropped)."""
+    s = _solver(meshes.parent_coords, meshes.parent_verts, case, scen)
+    e, f = meshes.faces_parent
+    rng = np.random.default_rng(seed)
+    Xs, Ys, Ss, obs = [], [], [], []
+    done = 0
+    while done < steps:
+        k = min(record_every, steps - done) if record_every > 0 else steps
+        if record_every > 0:
+            sel = np.arange(e.size) if face_fraction >= 1 else np.sort(
+                rng.choice(e.size, max(1, int(face_fraction * e.size)), replace=False))
+            tr, nrm = s.face_traces(e[sel], f[sel])
+            X, Y, S = dg.closure_features(tr, nrm, scen["v0"], case.rho0, case.c0)
+            Xs.append(X)
+            Ys.append(Y)
+            Ss.append(S)
+        r = s.run_steps(k, dt, t_start=done * dt, residuals=False)
+        obs.append(r.observers[:, :, 0])
+        done += k
+    s.close()
+    if not Xs:
+        return np.concatenate(obs), None, None
+    X, Y, S = np.concatenate(Xs), np.concatenate(Ys), np.concatenate(Ss)
+    # optionally drop faces the wave has not reached; targets of near-zero traces are clipped
+    keep = S > min_scale * S.max()
+    Y = np.clip(Y[keep], -target_clip, target_clip)
+    return np.concatenate(obs), X[keep], Y
+
+
+def truncated_run(case, meshes, scen, dt, steps, mode, params=None, precision=dg.DG_FP64):
+    """mode: 'characteristic' (the reference's absorbing BC) or 'learned' (params: BF16
+    network; None = the zero network, i.e. the exact characteristic condition)."""
+    K = meshes.sub_verts.shape[0]
+    bc = np.zeros((K, 3), dtype=np.int8)
+    if mode == "learned":
+        e, f = meshes.faces_sub
+        bc[e, f] = dg.DG_BC_LEARNED
+    s = _solver(meshes.sub_coords, meshes.sub_verts, case, scen, bc, precision)
+    if mode == "learned" and params is not None:
+        s.set_closure(params)
+    r = s.run_steps(steps, dt, residuals=False)
+    ms = r.device_ms
+    s.close()
+    return r.observers[:, :, 0], ms
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@dataclass
+class Report:
+    case: dict
+    dt: float
+    steps: int
+    train_samples: int
+    epoch_loss: list
+    train_seconds: float
+    alpha: float = 1.0
+    validation: dict = field(default_factory=dict)
+    eval: list = field(default_factory=list)
+
+
+def scale_output(master: np.ndarray, nfp: int, alpha: float) -> np.ndarray:
+    """Network with its output layer (W3, b3) multiplied by alpha: the learned incoming
+    characteristic is blended with the exact characteristic condition (alpha = 0)."""
+    nin, _ = dg.closure_sizes(nfp)
+    off = 64 * nin + 64 + 64 * 64 + 64
+    m = np.array(master, dtype=np.float32, copy=True)
+    m[off:] *= alpha
+    return m
+
+
+def train_and_evaluate(case: ClosureCase, n_train: int, n_eval: int, n_val: int = 2, epochs: int = 60,
+                       batch: int = 2048, lr: float = 1e-3, record_every: int = 2, face_fraction: float = 1.0,
+                       alphas=(0.0, 0.25, 0.5, 0.75, 1.0), log=print) -> Report:
+    """Training runs (seeds 1000+i) -> samples -> Adam on the GPU; the output blend alpha is
+    chosen on validation runs (seeds 3000+j) by the mean observer error; held-out runs
+    (seeds 5000+j) compare the reference's characteristic BC, the exact characteristic
+    condition (zero network) and the learned closure against the enlarged-domain solve."""
+    meshes = build_meshes(case)
+    dt, steps = time_step(case, meshes)
+    log(f"closure case: enlarged K={meshes.parent_verts.shape[0]} truncated K={meshes.sub_verts.shape[0]} "
+        f"truncation faces={meshes.faces_sub[0].size} dt={dt:.3e} steps={steps}")
+    Xs, Ys = [], []
+    for i in range(n_train):
+        scen = scenario(case, 1000 + i)
+        _, X, Y = enlarged_run(case, meshes, scen, dt, steps, record_every, face_fraction, seed=i)
+        Xs.append(X)
+        Ys.append(Y)
+        log(f"  training run {i + 1}/{n_train} ({scen['kind']}): {X.shape[0]} samples")
+    X = np.concatenate(Xs)
+    Y = np.concatenate(Ys)
+    del Xs, Ys
+    nfp = case.order + 1
+    master = dg.closure_init(nfp, seed=case.seed)
+    t0 = time.perf_counter()
+    master, _, hist = dg.closure_train(master, X, Y, epochs=epochs, batch=batch, lr=lr, seed=case.seed)
+    t_train = time.perf_counter() - t0
+    log(f"  trained {epochs} epochs on {X.shape[0]} samples in {t_train:.1f} s: loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    rep = Report(case=case.__dict__.copy(), dt=dt, steps=steps, train_samples=int(X.shape[0]),
+                 epoch_loss=[float(v) for v in hist], train_seconds=t_train)
+    del X, Y
+    # output blend chosen on validation runs
+    val = []
+    for j in range(n_val):
+        scen = scenario(case, 3000 + j)
+        ref, _, _ = enlarged_run(case, meshes, scen, dt, steps, record_every=0)
+        val.append((scen, ref))
+    best = None
+    for a in alphas:
+        w = dg.to_bf16(scale_output(master, nfp, a))
+        errs = [rel_l2(truncated_run(case, meshes, sc, dt, steps, "learned", w)[0], ref) for sc, ref in val]
+        rep.validation[str(a)] = errs
+        log(f"  validation alpha={a}: observer rel L2 {['%.3e' % e for e in errs]}")
+        if best is None or np.mean(errs) < best[0]:
+            best = (float(np.mean(errs)), a)
+    rep.alpha = best[1]
+    params = dg.to_bf16(scale_output(master, nfp, rep.alpha))
+    rep.params = params
+    rep.master = master
+    for j in range(n_eval):
+        scen = scenario(case, 5000 + j)
+        ref, _, _ = enlarged_run(case, meshes, scen, dt, steps, record_every=0)
+        out = dict(kind=scen["kind"], v0=[float(v) for v in scen["v0"]])
+        # characteristic: the reference's absorbing BC; exact_characteristic: the zero network
+        for name, mode, prm in (("characteristic", "characteristic", None),
+                                ("exact_characteristic", "learned", None), ("learned", "learned", params)):
+            o, ms = truncated_run(case, meshes, scen, dt, steps, mode, prm)
+            out[name] = rel_l2(o, ref)
+            out[name + "_ms_per_step"] = ms / steps
+        rep.eval.append(out)
+        log(f"  held-out {j + 1}/{n_eval} ({scen['kind']}): observer rel L2 vs enlarged: characteristic "
+            f"{out['characteristic']:.3e} exact-characteristic {out['exact_characteristic']:.3e} "
+            f"learned {out['learned']:.3e}")
+    return rep
+
+
+def main(argv=None):
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="Train the learned absorbing closure on enlarged-domain runs and "
+                                             "compare it with the characteristic BC on the truncated domain.")
+    ap.add_argument("--n-trunc", type=int, default=100, help="cells per side of [-1,1]^2 (h = 2/n)")
+    ap.add_argument("--order", type=int, default=4)
+    ap.add_argument("--factor", type=int, default=3, help="enlarged domain [-factor, factor]^2")
+    ap.add_argument("--train", type=int, default=32)
+    ap.add_argument("--val", type=int, default=2)
+    ap.add_argument("--eval", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--record-every", type=int, default=2)
+    ap.add_argument("--face-fraction", type=float, default=1.0)
+    ap.add_argument("--observer-size", type=float, default=0.03)
+    ap.add_argument("--out", default="", help="write the report as JSON here")
+    a = ap.parse_args(argv)
+    case = ClosureCase(n_trunc=a.n_trunc, order=a.order, factor=a.factor, observer_size=a.observer_size)
+    rep = train_and_evaluate(case, a.train, a.eval, a.val, a.epochs, a.batch, a.lr, a.record_every, a.face_fraction,
+                             log=lambda s: print(s, flush=True))
+    out = dict(rep.__dict__)
+    out.pop("params", None)
+    out.pop("master", None)
+    for k in ("characteristic", "exact_characteristic", "learned"):
+        out["mean_" + k] = float(np.mean([e[k] for e in rep.eval]))
+    print(json.dumps({k: out[k] for k in out if k.startswith("mean_") or k in ("alpha", "steps", "dt",
+                                                                              "train_samples")}), flush=True)
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

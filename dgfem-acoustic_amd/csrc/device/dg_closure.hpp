@@ -1,0 +1,411 @@
+// Learned absorbing-boundary closure (no counterpart in the reference solver; its
+// characteristic absorbing flux, src/Mesh.cpp:619-666 and :954-969, stays the fallback).
+//
+// On a truncation face the exterior state of the interior numerical flux
+// n.{F} + (c0/2)(q- - q+) is built so that the flux carries a prescribed incoming
+// characteristic predicted by a small MLP from the outgoing part of the interior trace.
+// 2D faces, face frame n (outward unit normal), t = (-n_y, n_x); per face node j:
+//   P = p / (rho0 c0), N = v.n, T = v.t; characteristics of the normal flux Jacobian:
+//   w_out = P + N (speed vn0 + c0), w_in = P - N (speed vn0 - c0), T (speed vn0)
+//   scale s = sqrt(mean_j (w_out^2 + T^2)) + 1e-30
+//   x = [w_out/s, T/s, vn0/c0, vt0/c0]                      (n_in = 2 Nfp + 2)
+//   y = W3 tanh(W2 tanh(W1 x + b1) + b2) + b3               (n_out = Nfp), w_hat = s y
+// Exterior state (upwind flux with incoming values w_hat and 0):
+//   w_out+ = w_out-
+//   w_in+  = (2 lam w_hat - (lam + c0) w_in-) / (lam - c0),  lam = vn0 - c0
+//   T+     = T- (vn0 >= 0);  -(vn0 + c0) T- / (vn0 - c0) (vn0 < 0: no incoming vorticity)
+//   p+ = rho0 c0 (w_out+ + w_in+)/2,  v+ = (w_out+ - w_in+)/2 n + T+ t,  vz+ = vz-,
+//   rho+ = rho- + (p+ - p-)/c0^2
+// The network never sees w_in-, so it cannot feed the interior's incoming content back in;
+// the zero network is the exact characteristic (first-order non-reflecting) condition. The
+// map is homogeneous of degree one in the trace and equivariant under rotations of the
+// face. Weights are BF16, activations and accumulation FP32. Parameter order (row-major,
+// out x in): W1[64][n_in], b1[64], W2[64][64], b2[64], W3[n_out][64], b3[n_out].
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "dg_kernels.hpp"
+
+namespace dgc {
+
+constexpr int kH = 64;  // hidden width
+
+__host__ __device__ constexpr int numParams(int nin, int nout) {
+    return kH * nin + kH + kH * kH + kH + nout * kH + nout;
+}
+
+__host__ __device__ inline float bf16ToFloat(uint16_t h) {
+    union {
+        uint32_t u;
+        float f;
+    } v;
+    v.u = static_cast<uint32_t>(h) << 16;
+    return v.f;
+}
+
+// Round to nearest even (NaN kept quiet).
+__host__ __device__ inline uint16_t floatToBf16(float f) {
+    union {
+        float f;
+        uint32_t u;
+    } v;
+    v.f = f;
+    if ((v.u & 0x7f800000u) == 0x7f800000u && (v.u & 0x7fffffu)) return static_cast<uint16_t>((v.u >> 16) | 0x40);
+    const uint32_t lsb = (v.u >> 16) & 1u;
+    return static_cast<uint16_t>((v.u + 0x7fffu + lsb) >> 16);
+}
+
+// LDS image of the network in FP32 with odd row strides (conflict-free row and column
+// walks across a wave).
+template <int NIN, int NOUT>
+struct NetLds {
+    static constexpr int S1 = NIN | 1;  // W1 row stride
+    static constexpr int S2 = kH + 1;   // W2 / W3 row stride
+    static constexpr int kW1 = 0;
+    static constexpr int kB1 = kW1 + kH * S1;
+    static constexpr int kW2 = kB1 + kH;
+    static constexpr int kB2 = kW2 + kH * S2;
+    static constexpr int kW3 = kB2 + kH;
+    static constexpr int kB3 = kW3 + NOUT * S2;
+    static constexpr int kTotal = kB3 + NOUT;
+};
+
+// Block-cooperative load of BF16 parameters (canonical order) into the LDS image.
+template <int NIN, int NOUT>
+__device__ inline void loadNet(const uint16_t* __restrict__ w, float* s, int tid, int nthreads) {
+    using N = NetLds<NIN, NOUT>;
+    constexpr int P = numParams(NIN, NOUT);
+    for (int x = tid; x < P; x += nthreads) {
+        const float v = bf16ToFloat(w[x]);
+        int o;
+        if (x < kH * NIN) {
+            o = N::kW1 + (x / NIN) * N::S1 + x % NIN;
+        } else if (x < kH * NIN + kH) {
+            o = N::kB1 + (x - kH * NIN);
+        } else if (x < kH * NIN + kH + kH * kH) {
+            const int y = x - (kH * NIN + kH);
+            o = N::kW2 + (y / kH) * N::S2 + y % kH;
+        } else if (x < kH * NIN + 2 * kH + kH * kH) {
+            o = N::kB2 + (x - (kH * NIN + kH + kH * kH));
+        } else if (x < kH * NIN + 2 * kH + kH * kH + NOUT * kH) {
+            const int y = x - (kH * NIN + 2 * kH + kH * kH);
+            o = N::kW3 + (y / kH) * N::S2 + y % kH;
+        } else {
+            o = N::kB3 + (x - (kH * NIN + 2 * kH + kH * kH + NOUT * kH));
+        }
+        s[o] = v;
+    }
+}
+
+__device__ inline void waveFence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Forward pass of one sample by one wave. sx[NIN] holds the input; on return sa1/sa2 hold
+// the hidden activations and sy[NOUT] the output.
+template <int NIN, int NOUT>
+__device__ inline void mlpForwardWave(const float* net, const float* sx, float* sa1, float* sa2, float* sy, int lane) {
+    using N = NetLds<NIN, NOUT>;
+    {
+        float z = net[N::kB1 + lane];
+        const float* w = net + N::kW1 + lane * N::S1;
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) z = fmaf(w[i], sx[i], z);
+        sa1[lane] = tanhf(z);
+    }
+    waveFence();
+    {
+        float z = net[N::kB2 + lane];
+        const float* w = net + N::kW2 + lane * N::S2;
+#pragma unroll 16
+        for (int k = 0; k < kH; ++k) z = fmaf(w[k], sa1[k], z);
+        sa2[lane] = tanhf(z);
+    }
+    waveFence();
+    if (lane < NOUT) {
+        float z = net[N::kB3 + lane];
+        const float* w = net + N::kW3 + lane * N::S2;
+#pragma unroll 16
+        for (int k = 0; k < kH; ++k) z = fmaf(w[k], sa2[k], z);
+        sy[lane] = z;
+    }
+    waveFence();
+}
+
+template <typename T>
+struct ClosureArgs {
+    const T* __restrict__ qin;          // stage input [K][NP][NF]
+    T* __restrict__ ghost;              // [nfaces][NFP][NF]
+    const T* __restrict__ geo;          // [K][4] dxi_r/dx_c
+    const T* __restrict__ ops;          // operator table (reference normals)
+    const int32_t* __restrict__ iops;   // fmask
+    const int64_t* __restrict__ faces;  // element * NFC + local face
+    const uint16_t* __restrict__ w;     // BF16 parameters
+    int64_t nfaces;
+    dgk::Physics<T> ph;
+};
+
+constexpr int kClosureFacesPerBlock = 4;
+
+// Exterior states of the learned faces for one stage: one wave per face, four per block.
+template <int P, int NF, typename T>
+__global__ __launch_bounds__(256) void closureKernel(ClosureArgs<T> a) {
+    using S = dgk::Sizes<2, P>;
+    using L = dgk::OpLayout<2, P>;
+    constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC;
+    constexpr int NIN = 2 * NFP + 2, NOUT = NFP;
+    using N = NetLds<NIN, NOUT>;
+    __shared__ float net[N::kTotal];
+    __shared__ float buf[kClosureFacesPerBlock][NIN + 2 * kH + NOUT + 2];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    loadNet<NIN, NOUT>(a.w, net, tid, 256);
+    __syncthreads();
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * kClosureFacesPerBlock + wave;
+    if (idx >= a.nfaces) return;  // whole wave leaves; no block barrier follows
+    float* sx = buf[wave];
+    float* sa1 = sx + NIN;
+    float* sa2 = sa1 + kH;
+    float* sy = sa2 + kH;
+    const int64_t ef = a.faces[idx];
+    const int64_t e = ef / NFC;
+    const int f = static_cast<int>(ef - e * NFC);
+    // outward unit normal
+    const T* J = a.geo + e * 4;
+    T n0 = J[0] * a.ops[L::kN + f * 3] + J[2] * a.ops[L::kN + f * 3 + 1];
+    T n1 = J[1] * a.ops[L::kN + f * 3] + J[3] * a.ops[L::kN + f * 3 + 1];
+    const T inv = T(1) / sqrt(n0 * n0 + n1 * n1);
+    n0 *= inv;
+    n1 *= inv;
+    const T t0 = -n1, t1 = n0;
+    const T vn0 = a.ph.v0[0] * n0 + a.ph.v0[1] * n1;
+    const T vt0 = a.ph.v0[0] * t0 + a.ph.v0[1] * t1;
+    T qm[NF], wo = T(0), wi = T(0), tt = T(0);
+    if (lane < NFP) {
+        const int node = a.iops[L::kMask + f * NFP + lane];
+        const T* q = a.qin + (e * NP + node) * NF;
+#pragma unroll
+        for (int k = 0; k < NF; ++k) qm[k] = q[k];
+        const T Pv = qm[0] * a.ph.irc0;
+        const T Nn = qm[1] * n0 + qm[2] * n1;
+        wo = Pv + Nn;
+        wi = Pv - Nn;
+        tt = qm[1] * t0 + qm[2] * t1;
+    }
+    const float wof = static_cast<float>(wo), ttf = static_cast<float>(tt);
+    float ss = wof * wof + ttf * ttf;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+    const float s = sqrtf(ss / NFP) + 1e-30f;
+    const float is = 1.0f / s;
+    if (lane < NFP) {
+        sx[lane] = wof * is;
+        sx[NFP + lane] = ttf * is;
+    }
+    if (lane == 0) {
+        const float ic0 = static_cast<float>(T(1) / a.ph.c0);
+        sx[2 * NFP] = static_cast<float>(vn0) * ic0;
+        sx[2 * NFP + 1] = static_cast<float>(vt0) * ic0;
+    }
+    waveFence();
+    mlpForwardWave<NIN, NOUT>(net, sx, sa1, sa2, sy, lane);
+    if (lane < NFP) {
+        const T c0 = a.ph.c0;
+        const T what = static_cast<T>(s * sy[lane]);
+        const T lam = vn0 - c0;
+        const T wip = (T(2) * lam * what - (lam + c0) * wi) / (lam - c0);
+        const T tp = (vn0 >= T(0)) ? tt : -(vn0 + c0) * tt / (vn0 - c0);
+        const T Pp = T(0.5) * (wo + wip), Np_ = T(0.5) * (wo - wip);
+        T qp[NF];
+        qp[0] = a.ph.rho0 * c0 * Pp;
+        qp[1] = Np_ * n0 + tp * t0;
+        qp[2] = Np_ * n1 + tp * t1;
+        qp[3] = qm[3];
+        if constexpr (NF == 5) qp[4] = qm[4] + (qp[0] - qm[0]) * a.ph.ic2;
+        T* g = a.ghost + (idx * NFP + lane) * NF;
+#pragma unroll
+        for (int k = 0; k < NF; ++k) g[k] = qp[k];
+    }
+}
+
+// Interior and exterior traces (exterior in the interior element's face-node order) and
+// outward unit normals of the listed faces: out [n][2][NFP][NF], normals [n][3].
+template <int DIM, int P, int NF, typename T>
+__global__ __launch_bounds__(256) void traceKernel(const T* __restrict__ q, const T* __restrict__ qlo,
+                                                   const T* __restrict__ geo, const T* __restrict__ ops,
+                                                   const int32_t* __restrict__ iops, const int32_t* __restrict__ nbr,
+                                                   const uint8_t* __restrict__ code, const int64_t* __restrict__ elem,
+                                                   const int32_t* __restrict__ face, int64_t n, double* __restrict__ out,
+                                                   double* __restrict__ normals) {
+    using S = dgk::Sizes<DIM, P>;
+    using L = dgk::OpLayout<DIM, P>;
+    constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC;
+    const int64_t it = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (it >= n * NFP) return;
+    const int64_t i = it / NFP;
+    const int an = static_cast<int>(it - i * NFP);
+    const int64_t e = elem[i];
+    const int f = face[i];
+    const int own = iops[L::kMask + f * NFP + an];
+    double* o = out + (i * 2 * NFP + an) * NF;
+    for (int k = 0; k < NF; ++k) {
+        const int64_t x = (e * NP + own) * NF + k;
+        o[k] = static_cast<double>(q[x]) + (qlo ? static_cast<double>(qlo[x]) : 0.0);
+    }
+    const int nb = nbr[e * NFC + f];
+    double* ox = out + (i * 2 * NFP + NFP + an) * NF;
+    if (nb >= 0) {
+        const int cd = code[e * NFC + f];
+        const int fp = cd & 3, pc = cd >> 2;
+        const int nn = iops[L::kMask + fp * NFP + iops[L::kPerm + pc * NFP + an]];
+        for (int k = 0; k < NF; ++k) {
+            const int64_t x = (static_cast<int64_t>(nb) * NP + nn) * NF + k;
+            ox[k] = static_cast<double>(q[x]) + (qlo ? static_cast<double>(qlo[x]) : 0.0);
+        }
+    } else {
+        for (int k = 0; k < NF; ++k) ox[k] = 0.0;
+    }
+    if (an == 0) {
+        const T* J = geo + e * DIM * DIM;
+        double nv[3] = {0, 0, 0}, nn2 = 0;
+        for (int c = 0; c < DIM; ++c) {
+            double s = 0;
+            for (int r = 0; r < DIM; ++r) s += static_cast<double>(J[r * DIM + c]) * static_cast<double>(ops[L::kN + f * 3 + r]);
+            nv[c] = s;
+            nn2 += s * s;
+        }
+        const double inv = 1.0 / sqrt(nn2);
+        for (int c = 0; c < 3; ++c) normals[i * 3 + c] = nv[c] * inv;
+    }
+}
+
+// ---------------------------------------------------------------- training kernels ----
+
+// Inference of n samples: X [n][NIN] -> Y [n][NOUT]; one wave per sample, grid-stride.
+template <int NIN, int NOUT>
+__global__ __launch_bounds__(64) void forwardKernel(const uint16_t* __restrict__ w, const float* __restrict__ X,
+                                                    float* __restrict__ Y, int64_t n) {
+    __shared__ float net[NetLds<NIN, NOUT>::kTotal];
+    __shared__ float sx[NIN], sa1[kH], sa2[kH], sy[NOUT];
+    const int lane = threadIdx.x;
+    loadNet<NIN, NOUT>(w, net, lane, 64);
+    __syncthreads();
+    for (int64_t s = blockIdx.x; s < n; s += gridDim.x) {
+        if (lane < NIN) sx[lane] = X[s * NIN + lane];
+        waveFence();
+        mlpForwardWave<NIN, NOUT>(net, sx, sa1, sa2, sy, lane);
+        if (lane < NOUT) Y[s * NOUT + lane] = sy[lane];
+        waveFence();
+    }
+}
+
+// Loss L = sum over the batch of |y - y*|^2 / (batch * NOUT) and its gradient. Each block
+// (one wave) takes samples [b*per, (b+1)*per) of the batch, in the order of idx (a
+// permutation of the data set, or null for identity), and writes its partial gradient
+// (canonical parameter order) and partial loss; partials are summed in block order.
+template <int NIN, int NOUT>
+__global__ __launch_bounds__(64) void gradKernel(const uint16_t* __restrict__ w, const float* __restrict__ X,
+                                                 const float* __restrict__ Yt, const int64_t* __restrict__ idx,
+                                                 int64_t first, int64_t count, int64_t per, float scale,
+                                                 float* __restrict__ gpart, double* __restrict__ lpart) {
+    using N = NetLds<NIN, NOUT>;
+    constexpr int P = numParams(NIN, NOUT);
+    __shared__ float net[N::kTotal];
+    __shared__ float sx[NIN], sa1[kH], sa2[kH], sy[NOUT], sd3[NOUT], sd2[kH];
+    const int lane = threadIdx.x;
+    loadNet<NIN, NOUT>(w, net, lane, 64);
+    __syncthreads();
+    float g1[NIN], g2[kH], g3[NOUT];
+    float gb1 = 0.f, gb2 = 0.f, gb3 = 0.f;
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) g1[i] = 0.f;
+#pragma unroll
+    for (int k = 0; k < kH; ++k) g2[k] = 0.f;
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o) g3[o] = 0.f;
+    double loss = 0.0;
+    const int64_t b0 = static_cast<int64_t>(blockIdx.x) * per;
+    const int64_t b1 = (b0 + per < count) ? b0 + per : count;
+    for (int64_t j = b0; j < b1; ++j) {
+        const int64_t s = idx ? idx[first + j] : first + j;
+        if (lane < NIN) sx[lane] = X[s * NIN + lane];
+        waveFence();
+        mlpForwardWave<NIN, NOUT>(net, sx, sa1, sa2, sy, lane);
+        if (lane < NOUT) {
+            const float e = sy[lane] - Yt[s * NOUT + lane];
+            loss += static_cast<double>(e) * static_cast<double>(e);
+            const float d3 = 2.0f * e * scale;
+            sd3[lane] = d3;
+            gb3 += d3;
+        }
+        waveFence();
+        // layer 3 weights (column `lane`) and the error at the second hidden layer
+        const float a2 = sa2[lane];
+        float gz2 = 0.f;
+#pragma unroll
+        for (int o = 0; o < NOUT; ++o) {
+            const float d3 = sd3[o];
+            g3[o] = fmaf(d3, a2, g3[o]);
+            gz2 = fmaf(net[N::kW3 + o * N::S2 + lane], d3, gz2);
+        }
+        const float d2 = gz2 * (1.0f - a2 * a2);
+        sd2[lane] = d2;
+        gb2 += d2;
+        waveFence();
+        // layer 2 weights (row `lane`) and the error at the first hidden layer
+        const float a1 = sa1[lane];
+        float gz1 = 0.f;
+#pragma unroll
+        for (int k = 0; k < kH; ++k) {
+            g2[k] = fmaf(d2, sa1[k], g2[k]);
+            gz1 = fmaf(net[N::kW2 + k * N::S2 + lane], sd2[k], gz1);
+        }
+        const float d1 = gz1 * (1.0f - a1 * a1);
+        gb1 += d1;
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) g1[i] = fmaf(d1, sx[i], g1[i]);
+        waveFence();
+    }
+    float* gp = gpart + static_cast<int64_t>(blockIdx.x) * P;
+    constexpr int oB1 = kH * NIN, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + NOUT * kH;
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) gp[lane * NIN + i] = g1[i];
+    gp[oB1 + lane] = gb1;
+#pragma unroll
+    for (int k = 0; k < kH; ++k) gp[oW2 + lane * kH + k] = g2[k];
+    gp[oB2 + lane] = gb2;
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o) gp[oW3 + o * kH + lane] = g3[o];
+    if (lane < NOUT) gp[oB3 + lane] = gb3;
+    // wave-level loss sum (lanes >= NOUT hold 0)
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
+    if (lane == 0) lpart[blockIdx.x] = loss;
+}
+
+// Sum of the block partials (in block order) -> grad, and optionally one Adam step on the
+// FP32 master parameters with a BF16 copy for the next forward pass.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void reduceAdamKernel(const float* __restrict__ gpart, int nparts, int P,
+                                                        float* __restrict__ grad, float* __restrict__ master,
+                                                        float* __restrict__ m1, float* __restrict__ m2,
+                                                        uint16_t* __restrict__ wbf, float lr, float beta1, float beta2,
+                                                        float eps, float bc1, float bc2, int doUpdate) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= P) return;
+    float g = 0.f;
+    for (int b = 0; b < nparts; ++b) g += gpart[static_cast<int64_t>(b) * P + x];
+    if (grad) grad[x] = g;
+    if (!doUpdate) return;
+    const float a = beta1 * m1[x] + (1.f - beta1) * g;
+    const float v = beta2 * m2[x] + (1.f - beta2) * g * g;
+    m1[x] = a;
+    m2[x] = v;
+    const float mh = a / bc1, vh = v / bc2;
+    const float p = master[x] - lr * mh / (sqrtf(vh) + eps);
+    master[x] = p;
+    wbf[x] = floatToBf16(p);
+}
+
+}  // namespace dgc

@@ -16,6 +16,7 @@
 #include "../../../include/dgfem_amd.h"
 #include "../host/mesh.hpp"
 #include "../host/refelem.hpp"
+#include "dg_closure.hpp"
 #include "dg_kernels.hpp"
 
 namespace {
@@ -32,6 +33,9 @@ struct TableSource {
 struct dg_context;
 using StageFn = void (*)(dg_context*, int stage, const void* qin, void* qout, void* uout, bool snap);
 using PerStepFn = void (*)(dg_context*);
+using ClosureFn = void (*)(dg_context*, const void* qin);
+using TraceFn = void (*)(dg_context*, int64_t n, const int64_t* elem, const int32_t* face, double* out,
+                         double* normals);
 
 struct dg_context {
     std::string err;
@@ -87,6 +91,14 @@ struct dg_context {
     PerStepFn sourceFn = nullptr, postFn = nullptr;
     int64_t deviceBytes = 0;
     double lastRunMs = 0.0;
+    // learned closure: faces, exterior states, BF16 network
+    int64_t nLearned = 0;
+    int64_t* dLearned = nullptr;
+    void* ghost = nullptr;
+    uint16_t* dClosureW = nullptr;
+    int64_t closureParams = 0;
+    ClosureFn closureFn = nullptr;
+    TraceFn traceFn = nullptr;
     // per-run settings read by the launchers
     int curResid = 1;
     double curDt = 0.0, gDt = -1.0;
@@ -139,6 +151,7 @@ void launchStage(dg_context* c, int stage, const void* qin, void* qout, void* uo
     a.acc = static_cast<T*>(c->acc);
     a.geo = static_cast<const T*>(c->geo);
     a.nbr = c->nbr;
+    a.ghost = static_cast<const T*>(c->ghost);
     a.code = c->code;
     a.ops = static_cast<const T*>(c->ops);
     a.iops = c->iops;
@@ -180,6 +193,7 @@ void launchStageWave(dg_context* c, int stage, const void* qin, void* qout, void
     a.acc = static_cast<T*>(c->acc);
     a.geo = static_cast<const T*>(c->geo);
     a.nbr = c->nbr;
+    a.ghost = static_cast<const T*>(c->ghost);
     a.code = c->code;
     a.ops = static_cast<const T*>(c->ops);
     a.iops = c->iops;
@@ -254,8 +268,45 @@ void launchPost(dg_context* c) {
                        p, sourceArgs(c), static_cast<T*>(c->u), static_cast<T*>(COMP ? c->ulo : nullptr));
 }
 
+template <int P, int NF, typename T>
+void launchClosure(dg_context* c, const void* qin) {
+    dgc::ClosureArgs<T> a{};
+    a.qin = static_cast<const T*>(qin);
+    a.ghost = static_cast<T*>(c->ghost);
+    a.geo = static_cast<const T*>(c->geo);
+    a.ops = static_cast<const T*>(c->ops);
+    a.iops = c->iops;
+    a.faces = c->dLearned;
+    a.w = c->dClosureW;
+    a.nfaces = c->nLearned;
+    a.ph = makePhysics<T>(c->phys);
+    const int64_t nb = (c->nLearned + dgc::kClosureFacesPerBlock - 1) / dgc::kClosureFacesPerBlock;
+    hipLaunchKernelGGL((dgc::closureKernel<P, NF, T>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, c->stream, a);
+}
+
+template <int DIM, int P, int NF, typename T>
+void launchTraces(dg_context* c, int64_t n, const int64_t* elem, const int32_t* face, double* out, double* normals) {
+    const int64_t threads = n * dgk::Sizes<DIM, P>::NFP;
+    const int64_t nb = (threads + 255) / 256;
+    hipLaunchKernelGGL((dgc::traceKernel<DIM, P, NF, T>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, c->stream,
+                       static_cast<const T*>(c->u), static_cast<const T*>(c->ulo), static_cast<const T*>(c->geo),
+                       static_cast<const T*>(c->ops), c->iops, c->nbr, c->code, elem, face, n, out, normals);
+}
+
+template <int DIM, int P, int NF>
+void pickAux(dg_context* c) {
+    if (c->precision == DG_FP64) {
+        c->traceFn = &launchTraces<DIM, P, NF, double>;
+        if constexpr (DIM == 2) c->closureFn = &launchClosure<P, NF, double>;
+    } else {
+        c->traceFn = &launchTraces<DIM, P, NF, float>;
+        if constexpr (DIM == 2) c->closureFn = &launchClosure<P, NF, float>;
+    }
+}
+
 template <int DIM, int P, int NF>
 bool pick(dg_context* c) {
+    pickAux<DIM, P, NF>(c);
     // Stage-kernel variant: DGFEM_KERNEL = valu (default) | wave (matrix-core, 4 fields).
     const char* env = std::getenv("DGFEM_KERNEL");
     const std::string kind = env ? env : "valu";
@@ -300,6 +351,12 @@ bool pickP(dg_context* c) {
         case 5: return pick<DIM, 5, NF>(c);
         default: return false;
     }
+}
+
+// One stage: exterior states of the learned faces from the stage input, then the stage.
+void runStage(dg_context* c, int stage, const void* qin, void* qout, void* uout, bool snap) {
+    if (c->nLearned > 0) c->closureFn(c, qin);
+    c->stageFn(c, stage, qin, qout, uout, snap);
 }
 
 bool pickKernels(dg_context* c) {
@@ -446,12 +503,12 @@ void syncTables(dg_context* c) {
 // Launch the kernels of one step on the context stream.
 void launchStep(dg_context* c, int integrator, bool resid) {
     if (integrator == DG_RK4) {
-        c->stageFn(c, dgk::kStageRK1, c->u, c->qa, c->u, false);
-        c->stageFn(c, dgk::kStageRK2, c->qa, c->qb, c->u, false);
-        c->stageFn(c, dgk::kStageRK3, c->qb, c->qa, c->u, false);
-        c->stageFn(c, dgk::kStageRK4, c->qa, nullptr, c->u, resid);
+        runStage(c, dgk::kStageRK1, c->u, c->qa, c->u, false);
+        runStage(c, dgk::kStageRK2, c->qa, c->qb, c->u, false);
+        runStage(c, dgk::kStageRK3, c->qb, c->qa, c->u, false);
+        runStage(c, dgk::kStageRK4, c->qa, nullptr, c->u, resid);
     } else {
-        c->stageFn(c, dgk::kStageEuler, c->u, nullptr, c->qb, resid);
+        runStage(c, dgk::kStageEuler, c->u, nullptr, c->qb, resid);
         HIPCHK(hipMemcpyAsync(c->u, c->qb, static_cast<size_t>(c->K) * c->Np * c->nf * c->tsize,
                               hipMemcpyDeviceToDevice, c->stream));
     }
@@ -550,6 +607,9 @@ void destroyContext(dg_context* c) {
     devFree(c->batchSteps);
     devFree(c->ticket);
     devFree(c->resPartial2);
+    devFree(c->dLearned);
+    devFree(c->ghost);
+    devFree(c->dClosureW);
     for (void* h : {static_cast<void*>(c->hT), static_cast<void*>(c->hV), static_cast<void*>(c->hRes),
                     static_cast<void*>(c->hObs), static_cast<void*>(c->hVF)})
         if (h) (void)hipHostFree(h);
@@ -602,6 +662,11 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         c->Nfp = R.Nfp;
         c->Nfc = R.Nfaces;
         if (c->K > INT32_MAX / 2) throw std::runtime_error("too many elements for 32-bit neighbour ids");
+        if (!c->mesh.learnedFaces.empty() && m->dim != 2) {
+            delete c;
+            g_createError = "dg_create: the learned closure (DG_BC_LEARNED) is built for 2D meshes";
+            return DG_ERR_UNSUPPORTED;
+        }
         if (!pickKernels(c)) {
             delete c;
             g_createError = "dg_create: no kernel built for this (dim, order)";
@@ -656,6 +721,14 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         c->iops = uploadVec(c, iops);
         c->nbr = uploadVec(c, c->mesh.nbr);
         c->code = uploadVec(c, c->mesh.nbrCode);
+        c->nLearned = static_cast<int64_t>(c->mesh.learnedFaces.size());
+        if (c->nLearned > 0) {
+            c->dLearned = uploadVec(c, c->mesh.learnedFaces);
+            c->ghost = alloc(static_cast<size_t>(c->nLearned) * c->Nfp * c->nf);
+            c->closureParams = dgc::numParams(2 * c->Nfp + 2, c->Nfp);
+            c->dClosureW = devAlloc<uint16_t>(c, c->closureParams);
+            HIPCHK(hipMemset(c->dClosureW, 0, c->closureParams * sizeof(uint16_t)));
+        }
         c->resPartial = devAlloc<double>(c, static_cast<size_t>(c->nblocks) * 5);
         HIPCHK(hipMemset(c->resPartial, 0, static_cast<size_t>(c->nblocks) * 5 * sizeof(double)));
         c->ringSlots = 4096;
@@ -683,6 +756,63 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
 }
 
 void dg_destroy(dg_context* c) { destroyContext(c); }
+
+dg_status dg_set_closure(dg_context* c, const uint16_t* params, int64_t n) {
+    if (!c || !params) return DG_ERR_INVALID;
+    if (c->nLearned == 0) return fail(c, DG_ERR_INVALID, "dg_set_closure: the mesh has no DG_BC_LEARNED faces");
+    if (n != c->closureParams)
+        return fail(c, DG_ERR_INVALID,
+                    "dg_set_closure: expected " + std::to_string(c->closureParams) + " parameters");
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(c->dClosureW, params, n * sizeof(uint16_t), hipMemcpyHostToDevice));
+    } catch (const std::exception& ex) {
+        return fail(c, DG_ERR_DEVICE, ex.what());
+    }
+    return DG_OK;
+}
+
+dg_status dg_learned_faces(const dg_context* c, int64_t* elem, int32_t* face, int64_t capacity, int64_t* num) {
+    if (!c || !num) return DG_ERR_INVALID;
+    *num = c->nLearned;
+    for (int64_t i = 0; i < c->nLearned && i < capacity; ++i) {
+        if (elem) elem[i] = c->mesh.learnedFaces[i] / c->Nfc;
+        if (face) face[i] = static_cast<int32_t>(c->mesh.learnedFaces[i] % c->Nfc);
+    }
+    return DG_OK;
+}
+
+dg_status dg_face_traces(dg_context* c, int64_t n, const int64_t* elem, const int32_t* face, double* out,
+                         double* normals) {
+    if (!c || n < 0 || (n > 0 && (!elem || !face || !out || !normals))) return DG_ERR_INVALID;
+    for (int64_t i = 0; i < n; ++i)
+        if (elem[i] < 0 || elem[i] >= c->K || face[i] < 0 || face[i] >= c->Nfc)
+            return fail(c, DG_ERR_INVALID, "dg_face_traces: element or face out of range");
+    if (n == 0) return DG_OK;
+    void *dE = nullptr, *dF = nullptr, *dO = nullptr, *dN = nullptr;
+    const size_t outN = static_cast<size_t>(n) * 2 * c->Nfp * c->nf;
+    dg_status st = DG_OK;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipMalloc(&dE, n * sizeof(int64_t)));
+        HIPCHK(hipMalloc(&dF, n * sizeof(int32_t)));
+        HIPCHK(hipMalloc(&dO, outN * sizeof(double)));
+        HIPCHK(hipMalloc(&dN, n * 3 * sizeof(double)));
+        HIPCHK(hipMemcpyAsync(dE, elem, n * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(dF, face, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        c->traceFn(c, n, static_cast<const int64_t*>(dE), static_cast<const int32_t*>(dF), static_cast<double*>(dO),
+                   static_cast<double*>(dN));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, dO, outN * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(normals, dN, n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const std::exception& ex) {
+        st = fail(c, DG_ERR_DEVICE, ex.what());
+    }
+    for (void* p : {dE, dF, dO, dN}) devFree(p);
+    return st;
+}
 
 const char* dg_last_error(const dg_context* c) { return c ? c->err.c_str() : g_createError.c_str(); }
 
@@ -953,7 +1083,7 @@ dg_status dg_rhs(dg_context* c, const double* u, double* k, double dt) {
         toDeviceLayout(c, u, dev);
         uploadState(c, dev, c->qa, nullptr);
         c->curDt = dt;
-        c->stageFn(c, dgk::kStageRHS, c->qa, c->qb, c->u, false);
+        runStage(c, dgk::kStageRHS, c->qa, c->qb, c->u, false);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
         std::vector<double> out;
@@ -1004,13 +1134,13 @@ dg_status dg_profile_steps(dg_context* c, double dt, int32_t steps, double* stag
             HIPCHK(hipEventRecord(ev[0], c->stream));
             c->sourceFn(c);
             HIPCHK(hipEventRecord(ev[1], c->stream));
-            c->stageFn(c, dgk::kStageRK1, c->u, c->qa, c->u, false);
+            runStage(c, dgk::kStageRK1, c->u, c->qa, c->u, false);
             HIPCHK(hipEventRecord(ev[2], c->stream));
-            c->stageFn(c, dgk::kStageRK2, c->qa, c->qb, c->u, false);
+            runStage(c, dgk::kStageRK2, c->qa, c->qb, c->u, false);
             HIPCHK(hipEventRecord(ev[3], c->stream));
-            c->stageFn(c, dgk::kStageRK3, c->qb, c->qa, c->u, false);
+            runStage(c, dgk::kStageRK3, c->qb, c->qa, c->u, false);
             HIPCHK(hipEventRecord(ev[4], c->stream));
-            c->stageFn(c, dgk::kStageRK4, c->qa, nullptr, c->u, true);
+            runStage(c, dgk::kStageRK4, c->qa, nullptr, c->u, true);
             HIPCHK(hipEventRecord(ev[5], c->stream));
             c->postFn(c);
             HIPCHK(hipEventRecord(ev[6], c->stream));

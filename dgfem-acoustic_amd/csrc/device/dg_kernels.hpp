@@ -64,7 +64,9 @@ struct StageArgs {
     T* __restrict__ ulo;         // compensation (mixed precision), else null
     T* __restrict__ acc;         // RK accumulator
     const T* __restrict__ geo;   // [K][DIM*DIM] dxi_r/dx_c
-    const int32_t* __restrict__ nbr;   // [K][NFC]
+    const int32_t* __restrict__ nbr;   // [K][NFC]: >= 0 neighbour, -1 absorbing, -2 reflecting,
+                                       // -3-i learned-closure face i (exterior state in ghost)
+    const T* __restrict__ ghost;       // [learned faces][NFP][NF] exterior states, or null
     const uint8_t* __restrict__ code;  // [K][NFC]
     const T* __restrict__ ops;
     const int32_t* __restrict__ iops;
@@ -228,11 +230,15 @@ __global__ __launch_bounds__(kBlock) void stageKernel(StageArgs<T> a) {
             const T* src = a.qin + (static_cast<int64_t>(nb) * NP + nn) * NF;
 #pragma unroll
             for (int k = 0; k < NF; ++k) qp[k] = src[k];
+        } else if (nb <= -3) {
+            const T* src = a.ghost + (static_cast<int64_t>(-3 - nb) * NFP + an) * NF;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) qp[k] = src[k];
         } else {
 #pragma unroll
             for (int k = 0; k < NF; ++k) qp[k] = T(0);
         }
-        numericalFlux<DIM, NF>(a.ph, nv, nb, qm, qp, pi);
+        numericalFlux<DIM, NF>(a.ph, nv, nb <= -3 ? 0 : nb, qm, qp, pi);
 #pragma unroll
         for (int k = 0; k < NF; ++k) sF[(el * NFT + fa) * NF + k] = fs * pi[k];
     }
@@ -559,10 +565,13 @@ __global__ __launch_bounds__(kBlock) void stageKernelWave(
                 const int nn = sI[L::kMask + fp * NFP + sI[L::kPerm + pc * NFP + an]];
                 const V4 v = reinterpret_cast<const V4*>(a.qin)[static_cast<int64_t>(nb) * NP + nn];
                 qp[0] = v.x; qp[1] = v.y; qp[2] = v.z; qp[3] = v.w;
+            } else if (nb <= -3) {
+                const V4 v = reinterpret_cast<const V4*>(a.ghost)[static_cast<int64_t>(-3 - nb) * NFP + an];
+                qp[0] = v.x; qp[1] = v.y; qp[2] = v.z; qp[3] = v.w;
             } else {
                 qp[0] = qp[1] = qp[2] = qp[3] = T(0);
             }
-            numericalFlux<DIM, NF>(a.ph, nv, nb, qm, qp, pi);
+            numericalFlux<DIM, NF>(a.ph, nv, nb <= -3 ? 0 : nb, qm, qp, pi);
             V4 out;
             out.x = fs * pi[0]; out.y = fs * pi[1]; out.z = fs * pi[2]; out.w = fs * pi[3];
             reinterpret_cast<V4*>(F)[fa * 4 + e] = out;
@@ -919,7 +928,7 @@ __global__ __launch_bounds__(256) void postKernel(PostArgs p, SourceArgs s, T* _
     }
 }
 
-__global__ void setCounterKernel(int64_t* counter, int64_t* batchSteps, int64_t nsteps, int* flag) {
+[[maybe_unused]] static __global__ void setCounterKernel(int64_t* counter, int64_t* batchSteps, int64_t nsteps, int* flag) {
     *counter = 0;
     *batchSteps = nsteps;
     *flag = -1;

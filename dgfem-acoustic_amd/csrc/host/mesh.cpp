@@ -383,7 +383,8 @@ DGMesh buildDGMesh(int dim, int order, int64_t K, const double* nodeCoords, cons
             int64_t e = keys[i].ef / nvpe;
             int f = int(keys[i].ef % nvpe);
             int8_t bc = faceBC ? faceBC[e * nvpe + f] : 0;
-            if (bc != 0 && bc != 1) throw std::runtime_error("boundary type must be 0 (absorbing) or 1 (reflecting)");
+            if (bc < 0 || bc > 2)
+                throw std::runtime_error("boundary type must be 0 (absorbing), 1 (reflecting) or 2 (learned)");
             m.nbr[e * nvpe + f] = -1 - bc;
             m.isBoundary[e * nvpe + f] = 1;
             ++nb;
@@ -410,6 +411,12 @@ DGMesh buildDGMesh(int dim, int order, int64_t K, const double* nodeCoords, cons
     }
     m.numFaces = nf;
     m.numBoundaryFaces = nb;
+    // learned-closure faces, numbered in (element, local face) order: nbr = -3 - index
+    for (int64_t ef = 0; ef < K * nvpe; ++ef)
+        if (m.nbr[ef] == -3) {
+            m.nbr[ef] = -3 - static_cast<int32_t>(m.learnedFaces.size());
+            m.learnedFaces.push_back(ef);
+        }
     return m;
 }
 

@@ -76,7 +76,9 @@ struct DGMesh {
     std::vector<double> Jinv;        // [K][3][3] dxi_r/dx_c (row r, column c); 2D uses the 2x2 block
     std::vector<double> detJ;        // [K]
     std::vector<double> inradius;    // [K]
-    std::vector<int32_t> nbr;        // [K][Nfaces]: neighbour element, or -1-bc on the boundary
+    std::vector<int32_t> nbr;        // [K][Nfaces]: neighbour element; boundary: -1 absorbing,
+                                     // -2 reflecting, -3-i learned-closure face i
+    std::vector<int64_t> learnedFaces;  // element * Nfaces + local face of each learned-closure face
     std::vector<uint8_t> nbrCode;    // [K][Nfaces]: neighbour local face | (perm code << 2)
     std::vector<uint8_t> isBoundary; // [K][Nfaces]
     int64_t numFaces = 0, numBoundaryFaces = 0;

@@ -20,7 +20,7 @@ DG_OK = 0
 DG_ERR_INVALID, DG_ERR_MESH, DG_ERR_DEVICE, DG_ERR_NONFINITE, DG_ERR_UNSUPPORTED, DG_ERR_CALLBACK = 1, 2, 3, 4, 5, 6
 DG_FP64, DG_MIXED = 0, 1
 DG_FIELDS_REF4, DG_FIELDS_LEE5 = 0, 1
-DG_BC_ABSORBING, DG_BC_REFLECTING = 0, 1
+DG_BC_ABSORBING, DG_BC_REFLECTING, DG_BC_LEARNED = 0, 1, 2
 DG_RK4, DG_EULER = 0, 1
 
 
@@ -53,6 +53,11 @@ class RunParams(C.Structure):
 class StepRecord(C.Structure):
     _fields_ = [("step", C.c_int64), ("t", C.c_double), ("residuals", C.POINTER(C.c_double)),
                 ("num_observers", C.c_int32), ("observers", C.POINTER(C.c_double)), ("step_seconds", C.c_double)]
+
+
+class TrainParams(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_double), ("beta1", C.c_double),
+                ("beta2", C.c_double), ("eps", C.c_double), ("seed", C.c_uint64)]
 
 
 class Info(C.Structure):
@@ -110,6 +115,21 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "dg_ref_mass": (C.c_int32, [C.c_int32, C.c_int32, P(C.c_double)]),
         "dg_mesh_precompute": (C.c_int32, [P(MeshDesc), P(C.c_double), P(C.c_int32), P(C.c_uint8), P(C.c_int32),
                                            P(C.c_int32)]),
+        "dg_set_closure": (C.c_int32, [C.c_void_p, P(C.c_uint16), C.c_int64]),
+        "dg_learned_faces": (C.c_int32, [C.c_void_p, P(C.c_int64), P(C.c_int32), C.c_int64, P(C.c_int64)]),
+        "dg_face_traces": (C.c_int32, [C.c_void_p, C.c_int64, P(C.c_int64), P(C.c_int32), P(C.c_double),
+                                       P(C.c_double)]),
+        "dg_closure_num_params": (C.c_int64, [C.c_int32, C.c_int32]),
+        "dg_closure_last_error": (C.c_char_p, []),
+        "dg_closure_to_bf16": (None, [P(C.c_float), C.c_int64, P(C.c_uint16)]),
+        "dg_closure_features": (C.c_int32, [C.c_int32, C.c_int32, C.c_int64, P(C.c_double), P(C.c_double),
+                                            P(Physics), P(C.c_float), P(C.c_float), P(C.c_float)]),
+        "dg_closure_forward": (C.c_int32, [C.c_int32, C.c_int32, P(C.c_uint16), P(C.c_float), C.c_int64,
+                                           P(C.c_float), C.c_int32]),
+        "dg_closure_loss_grad": (C.c_int32, [C.c_int32, C.c_int32, P(C.c_uint16), P(C.c_float), P(C.c_float),
+                                             C.c_int64, P(C.c_double), P(C.c_float), C.c_int32]),
+        "dg_closure_train": (C.c_int32, [C.c_int32, C.c_int32, P(C.c_float), P(C.c_uint16), P(C.c_float),
+                                         P(C.c_float), C.c_int64, P(TrainParams), P(C.c_double), C.c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -249,6 +269,108 @@ def mesh_precompute(node_coords, element_vertices, order, face_bc=None) -> dict:
     return dict(Jinv=Ji, nbr=nb, code=cd, face_perm=fp, to_canon=tc)
 
 
+# ------------------------------------------------------------ learned closure network ----
+
+def closure_sizes(nfp: int) -> tuple[int, int]:
+    """(n_in, n_out) of the closure network for faces with nfp nodes."""
+    return 2 * nfp + 2, nfp
+
+
+def closure_num_params(nfp: int) -> int:
+    nin, nout = closure_sizes(nfp)
+    return int(load_library().dg_closure_num_params(nin, nout))
+
+
+def to_bf16(x) -> np.ndarray:
+    """FP32 -> BF16 bit patterns (uint16), round to nearest even (library routine)."""
+    a = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    out = np.zeros(a.size, dtype=np.uint16)
+    load_library().dg_closure_to_bf16(_ptr(a, C.c_float), a.size, _ptr(out, C.c_uint16))
+    return out
+
+
+def _closure_check(st):
+    if st != DG_OK:
+        raise DGError(st, load_library().dg_closure_last_error().decode())
+
+
+def closure_features(traces, normals, v0, rho0, c0, with_targets=True):
+    """Features X [n][2nfp+2], targets Y [n][nfp] (or None) and scales s [n] of face traces
+    [n][2][nfp][nf] (interior, neighbour) with outward normals [n][3]."""
+    T = np.ascontiguousarray(traces, dtype=np.float64)
+    Nn = np.ascontiguousarray(normals, dtype=np.float64)
+    n, _, nfp, nf_ = T.shape
+    nin, nout = closure_sizes(nfp)
+    X = np.zeros((n, nin), dtype=np.float32)
+    Y = np.zeros((n, nout), dtype=np.float32) if with_targets else None
+    S = np.zeros(n, dtype=np.float32)
+    ph = Physics((C.c_double * 3)(*v0), rho0, c0, DG_FIELDS_LEE5 if nf_ == 5 else DG_FIELDS_REF4)
+    st = load_library().dg_closure_features(nfp, nf_, n, _ptr(T, C.c_double), _ptr(Nn, C.c_double), C.byref(ph),
+                                            _ptr(X, C.c_float), _ptr(Y, C.c_float) if Y is not None else None,
+                                            _ptr(S, C.c_float))
+    if st != DG_OK:
+        raise DGError(st, "dg_closure_features: invalid arguments")
+    return X, Y, S
+
+
+def closure_forward(params_bf16, X, device=0) -> np.ndarray:
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    n, nin = X.shape
+    nout = (nin - 2) // 2
+    W = np.ascontiguousarray(params_bf16, dtype=np.uint16)
+    Y = np.zeros((n, nout), dtype=np.float32)
+    _closure_check(load_library().dg_closure_forward(nin, nout, _ptr(W, C.c_uint16), _ptr(X, C.c_float), n,
+                                                     _ptr(Y, C.c_float), device))
+    return Y
+
+
+def closure_loss_grad(params_bf16, X, Y, device=0):
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.ascontiguousarray(Y, dtype=np.float32)
+    n, nin = X.shape
+    nout = Y.shape[1]
+    W = np.ascontiguousarray(params_bf16, dtype=np.uint16)
+    g = np.zeros(W.size, dtype=np.float32)
+    loss = C.c_double()
+    _closure_check(load_library().dg_closure_loss_grad(nin, nout, _ptr(W, C.c_uint16), _ptr(X, C.c_float),
+                                                       _ptr(Y, C.c_float), n, C.byref(loss), _ptr(g, C.c_float),
+                                                       device))
+    return loss.value, g
+
+
+def closure_init(nfp: int, seed: int = 0, out_scale: float = 0.0) -> np.ndarray:
+    """FP32 master parameters: Glorot-uniform hidden layers, output layer scaled by
+    out_scale (0: the network starts as the zero map, i.e. the characteristic condition)."""
+    nin, nout = closure_sizes(nfp)
+    rng = np.random.default_rng(seed)
+    H = 64
+
+    def glorot(o, i):
+        lim = np.sqrt(6.0 / (i + o))
+        return rng.uniform(-lim, lim, size=(o, i))
+
+    parts = [glorot(H, nin), np.zeros(H), glorot(H, H), np.zeros(H), out_scale * glorot(nout, H), np.zeros(nout)]
+    return np.concatenate([p.ravel() for p in parts]).astype(np.float32)
+
+
+def closure_train(master, X, Y, epochs=10, batch=4096, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, seed=1,
+                  device=0):
+    """Adam on the GPU (the library's forward/backward kernels). Returns (master FP32,
+    params BF16, per-epoch mean loss)."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.ascontiguousarray(Y, dtype=np.float32)
+    n, nin = X.shape
+    nout = Y.shape[1]
+    m = np.ascontiguousarray(master, dtype=np.float32).copy()
+    W = np.zeros(m.size, dtype=np.uint16)
+    hist = np.zeros(max(1, epochs))
+    prm = TrainParams(epochs, batch, lr, beta1, beta2, eps, seed)
+    _closure_check(load_library().dg_closure_train(nin, nout, _ptr(m, C.c_float), _ptr(W, C.c_uint16),
+                                                   _ptr(X, C.c_float), _ptr(Y, C.c_float), n, C.byref(prm),
+                                                   _ptr(hist, C.c_double), device))
+    return m, W, hist[:epochs]
+
+
 @dataclass
 class RunResult:
     steps: int
@@ -344,6 +466,33 @@ class Solver:
         self._check(self._lib.dg_add_observer(self._h, x, y, z, size, C.byref(n)))
         self.num_observers += 1
         return n.value
+
+    def set_closure(self, params_bf16):
+        """Network of the DG_BC_LEARNED faces (BF16 bit patterns, canonical order)."""
+        W = np.ascontiguousarray(params_bf16, dtype=np.uint16)
+        self._check(self._lib.dg_set_closure(self._h, _ptr(W, C.c_uint16), W.size))
+
+    def learned_faces(self):
+        """(element ids, local faces) of the learned faces in ghost order."""
+        n = C.c_int64()
+        self._check(self._lib.dg_learned_faces(self._h, None, None, 0, C.byref(n)))
+        e = np.zeros(n.value, dtype=np.int64)
+        f = np.zeros(n.value, dtype=np.int32)
+        self._check(self._lib.dg_learned_faces(self._h, _ptr(e, C.c_int64), _ptr(f, C.c_int32), n.value,
+                                               C.byref(n)))
+        return e, f
+
+    def face_traces(self, elem, face):
+        """Interior/neighbour traces [n][2][Nfp][nf] and outward normals [n][3] of the
+        current state on the faces (elem[i], face[i])."""
+        e = np.ascontiguousarray(elem, dtype=np.int64)
+        f = np.ascontiguousarray(face, dtype=np.int32)
+        nfp = self.info().nodes_per_face
+        out = np.zeros((e.size, 2, nfp, self.nf))
+        nrm = np.zeros((e.size, 3))
+        self._check(self._lib.dg_face_traces(self._h, e.size, _ptr(e, C.c_int64), _ptr(f, C.c_int32),
+                                             _ptr(out, C.c_double), _ptr(nrm, C.c_double)))
+        return out, nrm
 
     def info(self) -> Info:
         inf = Info()

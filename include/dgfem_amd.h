@@ -36,7 +36,9 @@ enum {
 
 enum { DG_FP64 = 0, DG_MIXED = 1 };                 /* precision */
 enum { DG_FIELDS_REF4 = 0, DG_FIELDS_LEE5 = 1 };    /* (p,vx,vy,vz) or + rho */
-enum { DG_BC_ABSORBING = 0, DG_BC_REFLECTING = 1 }; /* src/Mesh.cpp:575-578 */
+/* src/Mesh.cpp:575-578; DG_BC_LEARNED (2D only) has no counterpart in the reference: the
+   face takes its exterior state from the learned closure network (dg_set_closure). */
+enum { DG_BC_ABSORBING = 0, DG_BC_REFLECTING = 1, DG_BC_LEARNED = 2 };
 enum { DG_RK4 = 0, DG_EULER = 1 };                  /* config timeIntMethod */
 
 typedef struct dg_context dg_context;
@@ -144,6 +146,59 @@ DG_API dg_status dg_profile_steps(dg_context* ctx, double dt, int32_t steps, dou
 
 /* Wall time in ms (HIP events) of the device work of the last dg_run. */
 DG_API double dg_last_run_device_ms(const dg_context* ctx);
+
+/* ---- learned absorbing-boundary closure (no counterpart in the reference; its
+   characteristic absorbing BC, src/Mesh.cpp:619-666, stays the fallback) ----
+   Network: n_in = 2 Nfp + 2 -> 64 -> 64 -> n_out = Nfp, tanh, BF16 weights, FP32
+   accumulation; parameters in the order W1[64][n_in], b1[64], W2[64][64], b2[64],
+   W3[n_out][64], b3[n_out] (row-major, out x in). On a 2D face with outward normal n and
+   tangent t = (-n_y, n_x), per face node P = p/(rho0 c0), N = v.n, T = v.t and the
+   characteristics w_out = P + N, w_in = P - N: scale s = sqrt(mean(w_out^2 + T^2)) + 1e-30,
+   x = [w_out/s, T/s, v0.n/c0, v0.t/c0]; the network predicts the incoming characteristic
+   w_hat = s y, and the exterior state is chosen so that the face flux is the upwind flux
+   with incoming values w_hat (acoustic) and 0 (vorticity). The zero network is the exact
+   characteristic condition. */
+
+/* Network of the context's DG_BC_LEARNED faces (default: all zero, i.e. exterior state =
+   interior trace). params: BF16 bit patterns, n_params = dg_closure_num_params(...). */
+DG_API dg_status dg_set_closure(dg_context* ctx, const uint16_t* params, int64_t n_params);
+/* The learned faces in ghost order: element ids and local faces (capacity entries). */
+DG_API dg_status dg_learned_faces(const dg_context* ctx, int64_t* elem, int32_t* face, int64_t capacity,
+                                  int64_t* num_faces);
+/* Traces of the current state on faces (elem[i], face[i]): out [n][2][Nfp][fields] holds the
+   interior trace and the neighbour's trace in the interior face-node order (zeros on the
+   boundary); normals [n][3] the outward unit normals. */
+DG_API dg_status dg_face_traces(dg_context* ctx, int64_t n, const int64_t* elem, const int32_t* face, double* out,
+                                double* normals);
+
+DG_API int64_t dg_closure_num_params(int32_t n_in, int32_t n_out);
+DG_API const char* dg_closure_last_error(void);
+/* FP32 -> BF16 bit patterns, round to nearest even. */
+DG_API void dg_closure_to_bf16(const float* x, int64_t n, uint16_t* out);
+/* Features X [n][n_in], targets Y [n][n_out] (the neighbour trace's incoming characteristic
+   divided by s; may be NULL) and scales S [n] (may be NULL) from traces [n][2][nfp][nf] and
+   normals [n][3] as returned by dg_face_traces. */
+DG_API dg_status dg_closure_features(int32_t nfp, int32_t nf, int64_t n, const double* traces, const double* normals,
+                                     const dg_physics* phys, float* X, float* Y, float* S);
+/* Inference Y [n][n_out] of X [n][n_in] on `device`. */
+DG_API dg_status dg_closure_forward(int32_t n_in, int32_t n_out, const uint16_t* params, const float* X, int64_t n,
+                                    float* Y, int32_t device);
+/* Mean-square loss sum|y - Y|^2 / (n n_out) over all samples and its gradient grad[n_params]
+   (FP32, parameter order as above). */
+DG_API dg_status dg_closure_loss_grad(int32_t n_in, int32_t n_out, const uint16_t* params, const float* X,
+                                      const float* Y, int64_t n, double* loss, float* grad, int32_t device);
+typedef struct {
+    int32_t epochs;
+    int32_t batch;             /* samples per Adam step */
+    double lr, beta1, beta2, eps;
+    uint64_t seed;             /* shuffling */
+} dg_train_params;
+/* Adam on FP32 master parameters (in/out) with BF16 forward weights (out: params). The
+   forward and backward passes run on the GPU; epoch_loss[epochs] (may be NULL) receives
+   the mean loss of each epoch. */
+DG_API dg_status dg_closure_train(int32_t n_in, int32_t n_out, float* master, uint16_t* params, const float* X,
+                                  const float* Y, int64_t n, const dg_train_params* prm, double* epoch_loss,
+                                  int32_t device);
 
 /* ---- host-side mesh helpers (no device needed) ---- */
 

@@ -10,7 +10,9 @@ from __future__ import annotations
 import numpy as np
 
 
-def nodal_rhs(u, pre, ops, dim, v0, rho0, c0, dt, nf=4):
+def nodal_rhs(u, pre, ops, dim, v0, rho0, c0, dt, nf=4, closure=None):
+    """closure(q_int [m][Nfp][nf], normals [m][3]) -> exterior states of the learned
+    faces (neighbour code <= -3), used in the interior flux formula."""
     tc = pre["to_canon"]
     K, Np = tc.shape
     N = K * Np
@@ -61,6 +63,10 @@ def nodal_rhs(u, pre, ops, dim, v0, rho0, c0, dt, nf=4):
                 out[..., 4] = vn0[:, None] * qq[..., 4] + rho0 * nvq
             return out
 
+        learned = nb <= -3
+        if learned.any():
+            qp[learned] = closure(qm[learned], n[learned])
+        inner = inner | learned
         pi = np.zeros_like(qm)
         # interior
         pi[inner] = (0.5 * (fnorm(qm) + fnorm(qp)) + 0.5 * c0 * (qm - qp))[inner]
