@@ -140,8 +140,7 @@ template <typename T>
 struct ClosureArgs {
     const T* __restrict__ qin;          // stage input [K][NP][NF]
     T* __restrict__ ghost;              // [nfaces][NFP][NF]
-    const T* __restrict__ geo;          // [K][4] dxi_r/dx_c
-    const T* __restrict__ ops;          // operator table (reference normals)
+    const T* __restrict__ fgeo;         // [K][NFC][4] outward unit normal, Fscale
     const int32_t* __restrict__ iops;   // fmask
     const int64_t* __restrict__ faces;  // element * NFC + local face
     const uint16_t* __restrict__ w;     // BF16 parameters
@@ -174,12 +173,7 @@ __global__ __launch_bounds__(256) void closureKernel(ClosureArgs<T> a) {
     const int64_t e = ef / NFC;
     const int f = static_cast<int>(ef - e * NFC);
     // outward unit normal
-    const T* J = a.geo + e * 4;
-    T n0 = J[0] * a.ops[L::kN + f * 3] + J[2] * a.ops[L::kN + f * 3 + 1];
-    T n1 = J[1] * a.ops[L::kN + f * 3] + J[3] * a.ops[L::kN + f * 3 + 1];
-    const T inv = T(1) / sqrt(n0 * n0 + n1 * n1);
-    n0 *= inv;
-    n1 *= inv;
+    const T n0 = a.fgeo[(e * NFC + f) * 4], n1 = a.fgeo[(e * NFC + f) * 4 + 1];
     const T t0 = -n1, t1 = n0;
     const T vn0 = a.ph.v0[0] * n0 + a.ph.v0[1] * n1;
     const T vt0 = a.ph.v0[0] * t0 + a.ph.v0[1] * t1;

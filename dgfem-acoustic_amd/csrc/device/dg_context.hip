@@ -49,7 +49,7 @@ struct dg_context {
     dg_options opt{};
     // device state
     void *u = nullptr, *ulo = nullptr, *qa = nullptr, *qb = nullptr, *acc = nullptr, *snap = nullptr,
-         *snaplo = nullptr, *geo = nullptr, *ops = nullptr;
+         *snaplo = nullptr, *geo = nullptr, *fgeo = nullptr, *ops = nullptr;
     int32_t *nbr = nullptr, *iops = nullptr;
     uint8_t* code = nullptr;
     double* resPartial = nullptr;
@@ -150,6 +150,7 @@ void launchStage(dg_context* c, int stage, const void* qin, void* qout, void* uo
     a.ulo = COMP ? static_cast<T*>(c->ulo) : nullptr;
     a.acc = static_cast<T*>(c->acc);
     a.geo = static_cast<const T*>(c->geo);
+    a.fgeo = static_cast<const T*>(c->fgeo);
     a.nbr = c->nbr;
     a.ghost = static_cast<const T*>(c->ghost);
     a.code = c->code;
@@ -192,6 +193,7 @@ void launchStageWave(dg_context* c, int stage, const void* qin, void* qout, void
     a.ulo = COMP ? static_cast<T*>(c->ulo) : nullptr;
     a.acc = static_cast<T*>(c->acc);
     a.geo = static_cast<const T*>(c->geo);
+    a.fgeo = static_cast<const T*>(c->fgeo);
     a.nbr = c->nbr;
     a.ghost = static_cast<const T*>(c->ghost);
     a.code = c->code;
@@ -273,8 +275,7 @@ void launchClosure(dg_context* c, const void* qin) {
     dgc::ClosureArgs<T> a{};
     a.qin = static_cast<const T*>(qin);
     a.ghost = static_cast<T*>(c->ghost);
-    a.geo = static_cast<const T*>(c->geo);
-    a.ops = static_cast<const T*>(c->ops);
+    a.fgeo = static_cast<const T*>(c->fgeo);
     a.iops = c->iops;
     a.faces = c->dLearned;
     a.w = c->dClosureW;
@@ -591,7 +592,7 @@ void destroyContext(dg_context* c) {
     if (c->device >= 0) (void)hipSetDevice(c->device);
     releaseGraphs(c);
     freeTables(c);
-    for (void* p : {c->u, c->ulo, c->qa, c->qb, c->acc, c->snap, c->snaplo, c->geo, c->ops, c->afrag})
+    for (void* p : {c->u, c->ulo, c->qa, c->qb, c->acc, c->snap, c->snaplo, c->geo, c->fgeo, c->ops, c->afrag})
         devFree(p);
     devFree(c->nbr);
     devFree(c->iops);
@@ -704,6 +705,19 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
             for (int r = 0; r < c->dim; ++r)
                 for (int q = 0; q < c->dim; ++q) geo[(e * c->dim + r) * c->dim + q] = c->mesh.Jinv[e * 9 + r * 3 + q];
         c->geo = alloc(geo.size());
+        // per (element, face): outward unit normal and Fscale in FP64, rounded once
+        std::vector<double> fgeo(static_cast<size_t>(c->K) * c->Nfc * 4);
+        for (int64_t e = 0; e < c->K; ++e)
+            for (int f = 0; f < c->Nfc; ++f) {
+                double nv[3], fs;
+                dg::faceGeometry(c->mesh, e, f, nv, fs);
+                double* o = &fgeo[(static_cast<size_t>(e) * c->Nfc + f) * 4];
+                o[0] = nv[0];
+                o[1] = nv[1];
+                o[2] = nv[2];
+                o[3] = fs;
+            }
+        c->fgeo = alloc(fgeo.size());
         std::vector<double> ops;
         std::vector<int32_t> iops;
         buildOps(R, ops, iops);
@@ -711,10 +725,12 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         if (c->mfma) c->afrag = alloc(c->afragHost.size());
         if (c->precision == DG_FP64) {
             uploadAs<double>(c, c->geo, geo);
+            uploadAs<double>(c, c->fgeo, fgeo);
             uploadAs<double>(c, c->ops, ops);
             if (c->mfma) uploadAs<double>(c, c->afrag, c->afragHost);
         } else {
             uploadAs<float>(c, c->geo, geo);
+            uploadAs<float>(c, c->fgeo, fgeo);
             uploadAs<float>(c, c->ops, ops);
             if (c->mfma) uploadAs<float>(c, c->afrag, c->afragHost);
         }

@@ -64,6 +64,7 @@ struct StageArgs {
     T* __restrict__ ulo;         // compensation (mixed precision), else null
     T* __restrict__ acc;         // RK accumulator
     const T* __restrict__ geo;   // [K][DIM*DIM] dxi_r/dx_c
+    const T* __restrict__ fgeo;  // [K][NFC][4] outward unit normal, Fscale = |J^-T Nref| (= sJ/|J|)
     const int32_t* __restrict__ nbr;   // [K][NFC]: >= 0 neighbour, -1 absorbing, -2 reflecting,
                                        // -3-i learned-closure face i (exterior state in ghost)
     const T* __restrict__ ghost;       // [learned faces][NFP][NF] exterior states, or null
@@ -152,11 +153,18 @@ __global__ __launch_bounds__(kBlock) void stageKernel(StageArgs<T> a) {
     constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC, NFT = S::NFT, EB = S::EB;
     constexpr int JJ = DIM * DIM;
     constexpr bool kOpsInLds = (L::kTotal * sizeof(T)) <= 40 * 1024;
+    // FP32: per-face normals and neighbour codes staged in LDS once per block (measured faster);
+    // FP64: read per face node through the caches (the extra LDS costs more than it saves)
+    constexpr bool kStageFaces = sizeof(T) == 4;
+    constexpr int kFaceLds = kStageFaces ? EB * NFC : 1;
     constexpr int kOpsLds = kOpsInLds ? L::kTotal : 1;
 
     __shared__ T sQ[EB * NP * NF];
     __shared__ T sF[EB * NFT * NF];
     __shared__ T sJ[EB * JJ];
+    __shared__ T sN[kFaceLds * 4];       // per (element, face): outward unit normal, Fscale
+    __shared__ int32_t sNb[kFaceLds];    // per (element, face): neighbour / boundary code
+    __shared__ int32_t sCd[kFaceLds];    // per (element, face): neighbour face | perm << 2
     __shared__ T sOps[kOpsLds];
     __shared__ int32_t sI[L::kITotal];
     __shared__ double sRed[kBlock / 64][5];
@@ -197,34 +205,34 @@ __global__ __launch_bounds__(kBlock) void stageKernel(StageArgs<T> a) {
         const T* g = a.geo + e0 * JJ;
         for (int x = tid; x < nE * JJ; x += kBlock) sJ[x] = g[x];
     }
+    // per (element, face): unit normal, Fscale, neighbour and code, staged once for all the
+    // face nodes of the block
+    if constexpr (kStageFaces) {
+        const T* fg = a.fgeo + e0 * NFC * 4;
+        for (int x = tid; x < nE * NFC * 4; x += kBlock) sN[x] = fg[x];
+        for (int x = tid; x < nE * NFC; x += kBlock) {
+            sNb[x] = a.nbr[e0 * NFC + x];
+            sCd[x] = a.code[e0 * NFC + x];
+        }
+    }
     __syncthreads();
 
-    // ---- numerical fluxes at face nodes, scaled by Fscale = sJ/|J| ----
+    // ---- numerical fluxes at face nodes, scaled by Fscale ----
     for (int it = tid; it < nE * NFT; it += kBlock) {
         const int el = it / NFT, fa = it - el * NFT;
         const int f = fa / NFP, an = fa - f * NFP;
-        const int64_t eg = e0 + el;
-        const T* J = &sJ[el * JJ];
-        T nv[3] = {T(0), T(0), T(0)};
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) {
-            T s = T(0);
-#pragma unroll
-            for (int r = 0; r < DIM; ++r) s += J[r * DIM + c] * ops[L::kN + f * 3 + r];
-            nv[c] = s;
-        }
-        const T fs = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
-        const T ifs = T(1) / fs;
-        nv[0] *= ifs;
-        nv[1] *= ifs;
-        nv[2] *= ifs;
+        const int64_t efg = (e0 + el) * NFC + f;
+        const int efl = kStageFaces ? el * NFC + f : 0;
+        const T* nf = kStageFaces ? &sN[efl * 4] : &a.fgeo[efg * 4];
+        const T nv[3] = {nf[0], nf[1], nf[2]};
+        const T fs = nf[3];
         const int own = sI[L::kMask + f * NFP + an];
         T qm[NF], qp[NF], pi[NF];
 #pragma unroll
         for (int k = 0; k < NF; ++k) qm[k] = sQ[(el * NP + own) * NF + k];
-        const int nb = a.nbr[eg * NFC + f];
+        const int nb = kStageFaces ? sNb[efl] : a.nbr[efg];
         if (nb >= 0) {
-            const int cd = a.code[eg * NFC + f];
+            const int cd = kStageFaces ? sCd[efl] : a.code[efg];
             const int fp = cd & 3, pc = cd >> 2;
             const int nn = sI[L::kMask + fp * NFP + sI[L::kPerm + pc * NFP + an]];
             const T* src = a.qin + (static_cast<int64_t>(nb) * NP + nn) * NF;
@@ -482,7 +490,6 @@ __global__ __launch_bounds__(kBlock) void stageKernelWave(
 
     __shared__ T sA[M::kFragTotal];
     __shared__ int32_t sI[L::kITotal];
-    __shared__ T sNref[NFC * 3];
     __shared__ __align__(16) T sQ[4][4 * NP * NF];    // per wave: staged state of its group
     __shared__ __align__(16) T sF[4][M::KSL * 4 * 16];  // per wave: lift B rows (fa x 16 cols)
     __shared__ __align__(16) T sR[4][4 * NP * NF];    // per wave: rhs transpose
@@ -498,7 +505,6 @@ __global__ __launch_bounds__(kBlock) void stageKernelWave(
 
     for (int x = tid; x < M::kFragTotal; x += kBlock) sA[x] = afrag[x];
     for (int x = tid; x < L::kITotal; x += kBlock) sI[x] = a.iops[x];
-    for (int x = tid; x < NFC * 3; x += kBlock) sNref[x] = a.ops[L::kN + x];
     __syncthreads();
 
     const int col = lane & 15, cel = col >> 2, fld = col & 3, brow = lane >> 4;
@@ -525,24 +531,7 @@ __global__ __launch_bounds__(kBlock) void stageKernelWave(
         for (int x = lane; x < M::KSL * 4 * 16; x += 64) F[x] = T(0);
         waveSync();
         // -- unit normals and Fscale of the group's faces
-        if (lane < nE * NFC) {
-            const int e = lane / NFC, f = lane - e * NFC;
-            const T* J = &Jw[e * JJ];
-            T nv[3] = {T(0), T(0), T(0)};
-#pragma unroll
-            for (int c = 0; c < DIM; ++c) {
-                T sum = T(0);
-#pragma unroll
-                for (int r = 0; r < DIM; ++r) sum += J[r * DIM + c] * sNref[f * 3 + r];
-                nv[c] = sum;
-            }
-            const T fs = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
-            const T ifs = T(1) / fs;
-            Nw[lane * 4 + 0] = nv[0] * ifs;
-            Nw[lane * 4 + 1] = nv[1] * ifs;
-            Nw[lane * 4 + 2] = nv[2] * ifs;
-            Nw[lane * 4 + 3] = fs;
-        }
+        for (int x = lane; x < nE * NFC * 4; x += 64) Nw[x] = a.fgeo[e0 * NFC * 4 + x];
         waveSync();
         // -- numerical fluxes at the group's face nodes -> lift B rows
         for (int it = lane; it < nE * NFT; it += 64) {

@@ -14,7 +14,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libdgfem_amd.so")
+# DGFEM_LIB selects another build of the library (e.g. a variant kept beside the default).
+LIB_PATH = os.environ.get("DGFEM_LIB") or os.path.join(_HERE, "libdgfem_amd.so")
 
 DG_OK = 0
 DG_ERR_INVALID, DG_ERR_MESH, DG_ERR_DEVICE, DG_ERR_NONFINITE, DG_ERR_UNSUPPORTED, DG_ERR_CALLBACK = 1, 2, 3, 4, 5, 6
