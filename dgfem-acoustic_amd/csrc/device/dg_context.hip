@@ -33,6 +33,7 @@ struct TableSource {
 struct dg_context;
 using StageFn = void (*)(dg_context*, int stage, const void* qin, void* qout, void* uout, bool snap);
 using PerStepFn = void (*)(dg_context*);
+using OccFn = int (*)();
 using ClosureFn = void (*)(dg_context*, const void* qin);
 using TraceFn = void (*)(dg_context*, int64_t n, const int64_t* elem, const int32_t* face, double* out,
                          double* normals);
@@ -84,7 +85,10 @@ struct dg_context {
     int gSteps = 0, gIntegrator = -1, gResid = -1;
     // dispatch
     StageFn stageFn = nullptr;
-    int eb = 1;              // elements per block of the chosen stage kernel
+    int eb = 1;              // elements per block (batch) of the chosen stage kernel
+    OccFn occFn = nullptr;   // resident blocks per CU of a persistent stage kernel (else null):
+                             // FP64 (persistent blocks stage the operator table once; measured
+                             // faster), FP32 runs one batch per block (measured faster)
     bool mfma = false;       // matrix-core stage kernel in use
     void* afrag = nullptr;   // A-operand fragments of the matrix-core kernel
     std::vector<double> afragHost;
@@ -210,6 +214,52 @@ void launchStageWave(dg_context* c, int stage, const void* qin, void* qout, void
                        dim3(dgk::kBlock), 0, c->stream, a, static_cast<const T*>(c->afrag));
 }
 
+template <int DIM, int P, typename T, bool COMP>
+void launchStageMM(dg_context* c, int stage, const void* qin, void* qout, void* uout, bool snap) {
+    dgk::StageArgs<T> a{};
+    a.qin = static_cast<const T*>(qin);
+    a.qout = static_cast<T*>(qout);
+    a.u = static_cast<T*>(c->u);
+    a.uout = static_cast<T*>(uout);
+    a.ulo = COMP ? static_cast<T*>(c->ulo) : nullptr;
+    a.acc = static_cast<T*>(c->acc);
+    a.geo = static_cast<const T*>(c->geo);
+    a.fgeo = static_cast<const T*>(c->fgeo);
+    a.nbr = c->nbr;
+    a.ghost = static_cast<const T*>(c->ghost);
+    a.code = c->code;
+    a.ops = static_cast<const T*>(c->ops);
+    a.iops = c->iops;
+    a.snap = snap ? static_cast<const T*>(c->snap) : nullptr;
+    a.snaplo = (snap && COMP) ? static_cast<const T*>(c->snaplo) : nullptr;
+    a.resPartial = c->resPartial;
+    a.K = c->K;
+    a.dt = static_cast<T>(c->curDt);
+    a.stage = stage;
+    a.ph = makePhysics<T>(c->phys);
+    hipLaunchKernelGGL((dgk::stageKernelMM<DIM, P, T, COMP>), dim3(static_cast<unsigned>(c->nblocks)),
+                       dim3(dgk::kBlock), 0, c->stream, a, static_cast<const T*>(c->afrag));
+}
+
+template <int DIM, int P>
+void buildAfrag(dg_context* c, const dg::RefElement& R);
+
+// The same fragments lane-contiguous ([lane][mt][ks], padded to a multiple of 4 per lane) for
+// the block matrix-core kernel.
+template <int DIM, int P>
+void buildAfragLane(dg_context* c, const dg::RefElement& R) {
+    using M = dgk::MfmaShape<DIM, P>;
+    using B = dgk::BlockMmShape<DIM, P>;
+    buildAfrag<DIM, P>(c, R);
+    std::vector<double> lanes(static_cast<size_t>(64) * B::kAPerLane, 0.0);
+    for (int lane = 0; lane < 64; ++lane)
+        for (int mt = 0; mt < M::MT; ++mt)
+            for (int ks = 0; ks < M::KS; ++ks)
+                lanes[static_cast<size_t>(lane) * B::kAPerLane + mt * M::KS + ks] =
+                    c->afragHost[(mt * M::KS + ks) * 64 + lane];
+    c->afragHost.swap(lanes);
+}
+
 // A-operand fragments [MT][KSV+KSL][64 lanes] of the matrix-core kernel: row 16*mt + lane%16,
 // k = 4*ks + lane/16 of [Dw_1 | .. | Dw_dim] (volume k-steps) then of -LIFT (lift k-steps).
 template <int DIM, int P>
@@ -270,6 +320,15 @@ void launchPost(dg_context* c) {
                        p, sourceArgs(c), static_cast<T*>(c->u), static_cast<T*>(COMP ? c->ulo : nullptr));
 }
 
+template <int DIM, int P, int NF, typename T, bool COMP>
+int stageOccupancy() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dgk::stageKernel<DIM, P, NF, T, COMP>, dgk::kBlock, 0) !=
+        hipSuccess)
+        n = 0;
+    return n;
+}
+
 template <int P, int NF, typename T>
 void launchClosure(dg_context* c, const void* qin) {
     dgc::ClosureArgs<T> a{};
@@ -308,9 +367,27 @@ void pickAux(dg_context* c) {
 template <int DIM, int P, int NF>
 bool pick(dg_context* c) {
     pickAux<DIM, P, NF>(c);
-    // Stage-kernel variant: DGFEM_KERNEL = valu (default) | wave (matrix-core, 4 fields).
+    // Stage-kernel variant: DGFEM_KERNEL = valu (default) | mm (block matrix-core, 4 fields)
+    // | wave (wave-autonomous matrix-core, 4 fields).
     const char* env = std::getenv("DGFEM_KERNEL");
     const std::string kind = env ? env : "valu";
+    if constexpr (NF == 4 && dgk::BlockMmShape<DIM, P>::kSupported) {
+        if (kind == "mm") {
+            c->mfma = true;
+            buildAfragLane<DIM, P>(c, dg::refElement(DIM, P));
+            c->eb = dgk::BlockMmShape<DIM, P>::EB;
+            if (c->precision == DG_FP64) {
+                c->stageFn = &launchStageMM<DIM, P, double, false>;
+                c->sourceFn = &launchSource<double, false>;
+                c->postFn = &launchPost<double, false>;
+            } else {
+                c->stageFn = &launchStageMM<DIM, P, float, true>;
+                c->sourceFn = &launchSource<float, true>;
+                c->postFn = &launchPost<float, true>;
+            }
+            return true;
+        }
+    }
     if constexpr (NF == 4 && dgk::MfmaShape<DIM, P>::kSupported) {
         if (kind == "wave") {
             c->mfma = true;
@@ -331,6 +408,7 @@ bool pick(dg_context* c) {
     c->mfma = false;
     c->eb = dgk::Sizes<DIM, P>::EB;
     if (c->precision == DG_FP64) {
+        c->occFn = &stageOccupancy<DIM, P, NF, double, false>;
         c->stageFn = &launchStage<DIM, P, NF, double, false>;
         c->sourceFn = &launchSource<double, false>;
         c->postFn = &launchPost<double, false>;
@@ -683,6 +761,14 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         HIPCHK(hipSetDevice(c->device));
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->nblocks = (c->K + c->eb - 1) / c->eb;
+        if (c->occFn) {
+            // persistent stage kernel: one block per resident slot (a multiple of the 8 XCDs)
+            int cus = 0;
+            HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+            const int64_t slots = static_cast<int64_t>(cus) * std::max(1, c->occFn());
+            if (slots < c->nblocks) c->nblocks = slots;
+            if (c->nblocks >= 8) c->nblocks -= c->nblocks % 8;
+        }
         const size_t nstate = static_cast<size_t>(c->K) * c->Np * c->nf;
         auto alloc = [&](size_t n) -> void* {
             return c->precision == DG_FP64 ? static_cast<void*>(devAlloc<double>(c, n))

@@ -146,8 +146,16 @@ __device__ inline void twoSum(T a, T b, T& s, T& e) {
     e = (a - (s - bb)) + (b - bb);
 }
 
+// FP32 with 4 fields and 10..20 nodes per element asks for 7 waves per SIMD (keeps the
+// single-batch kernel at <= 72 VGPRs; measured faster); other variants are left to the compiler.
+template <int DIM, int P, int NF, typename T>
+struct StageWaves {
+    static constexpr int NP = Sizes<DIM, P>::NP;
+    static constexpr int value = (sizeof(T) == 4 && NF == 4 && NP >= 10 && NP <= 20) ? 7 : 1;
+};
+
 template <int DIM, int P, int NF, typename T, bool COMP>
-__global__ __launch_bounds__(kBlock) void stageKernel(StageArgs<T> a) {
+__global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void stageKernel(StageArgs<T> a) {
     using S = Sizes<DIM, P>;
     using L = OpLayout<DIM, P>;
     constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC, NFT = S::NFT, EB = S::EB;
@@ -158,6 +166,10 @@ __global__ __launch_bounds__(kBlock) void stageKernel(StageArgs<T> a) {
     constexpr bool kStageFaces = sizeof(T) == 4;
     constexpr int kFaceLds = kStageFaces ? EB * NFC : 1;
     constexpr int kOpsLds = kOpsInLds ? L::kTotal : 1;
+    // FP64: persistent blocks loop over batches (the operator table is staged once per block;
+    // measured faster). FP32: one batch per block (higher occupancy; measured faster). The
+    // host sizes the grid accordingly.
+    constexpr bool kPersistent = sizeof(T) == 8;
 
     __shared__ T sQ[EB * NP * NF];
     __shared__ T sF[EB * NFT * NF];
@@ -170,15 +182,33 @@ __global__ __launch_bounds__(kBlock) void stageKernel(StageArgs<T> a) {
     __shared__ double sRed[kBlock / 64][5];
 
     const int tid = threadIdx.x;
-    const int64_t blk = xcdRemap(blockIdx.x, gridDim.x);
-    const int64_t e0 = blk * EB;
+    // constant tables, once per block
+    if constexpr (kOpsInLds)
+        for (int x = tid; x < L::kTotal; x += kBlock) sOps[x] = a.ops[x];
+    for (int x = tid; x < L::kITotal; x += kBlock) sI[x] = a.iops[x];
+    const T* ops = kOpsInLds ? sOps : a.ops;
+    double res[5] = {0, 0, 0, 0, 0};
+
+    // batches of EB elements: each XCD takes a contiguous range, strided over its blocks (one
+    // batch per block when the grid covers all batches; persistent blocks otherwise)
+    const int64_t nBatches = (a.K + EB - 1) / EB;
+    int64_t bFirst = blockIdx.x, bEnd = nBatches, bStep = gridDim.x;
+    if (gridDim.x % 8 == 0) {
+        const int64_t xcd = blockIdx.x % 8, perXcd = gridDim.x / 8;
+        bFirst = nBatches * xcd / 8 + blockIdx.x / 8;
+        bEnd = nBatches * (xcd + 1) / 8;
+        bStep = perXcd;
+    }
+    const bool needU = a.stage == kStageRK2 || a.stage == kStageRK3 || a.stage == kStageRK4;
+    const bool isFinal = a.stage == kStageRK4 || a.stage == kStageEuler;
+    // one batch of EB elements (inlined into either the persistent loop or a single call)
+    auto batch = [&](int64_t bi) {
+    const int64_t e0 = bi * EB;
     const int nE = static_cast<int>((a.K - e0) < EB ? (a.K - e0) : EB);
 
     // prefetch the RK operands of this thread's node (consumed after the contractions)
     const bool active = tid < nE * NP;
     const int64_t myBase = (e0 * NP + tid) * NF;
-    const bool needU = a.stage == kStageRK2 || a.stage == kStageRK3 || a.stage == kStageRK4;
-    const bool isFinal = a.stage == kStageRK4 || a.stage == kStageEuler;
     T pu[NF], pacc[NF], plo[NF], ps[NF], psl[NF];
 #pragma unroll
     for (int k = 0; k < NF; ++k) {
@@ -195,10 +225,6 @@ __global__ __launch_bounds__(kBlock) void stageKernel(StageArgs<T> a) {
             }
         }
     }
-    if constexpr (kOpsInLds)
-        for (int x = tid; x < L::kTotal; x += kBlock) sOps[x] = a.ops[x];
-    for (int x = tid; x < L::kITotal; x += kBlock) sI[x] = a.iops[x];
-    const T* ops = kOpsInLds ? sOps : a.ops;
     {
         const T* src = a.qin + e0 * NP * NF;
         for (int x = tid; x < nE * NP * NF; x += kBlock) sQ[x] = src[x];
@@ -253,7 +279,6 @@ __global__ __launch_bounds__(kBlock) void stageKernel(StageArgs<T> a) {
     __syncthreads();
 
     // ---- volume contraction, flux-Jacobian fold, lift, register update ----
-    double res[5] = {0, 0, 0, 0, 0};
     if (tid < nE * NP) {
         const int el = tid / NP, i = tid - el * NP;
         const int64_t node = (e0 + el) * NP + i;
@@ -378,13 +403,22 @@ __global__ __launch_bounds__(kBlock) void stageKernel(StageArgs<T> a) {
                 const double ic2 = static_cast<double>(a.ph.ic2);
                 const double u0 = static_cast<double>(unew[0]) + lo[0];
                 const double sp = sv[0] * ic2 - u0 * ic2;
-                res[0] = d[0] * d[0];
-                res[1] = sp * sp;
-                res[2] = d[1] * d[1];
-                res[3] = d[2] * d[2];
-                res[4] = d[3] * d[3];
+                res[0] += d[0] * d[0];
+                res[1] += sp * sp;
+                res[2] += d[1] * d[1];
+                res[3] += d[2] * d[2];
+                res[4] += d[3] * d[3];
             }
         }
+    }
+    };
+    if constexpr (kPersistent) {
+        for (int64_t bi = bFirst; bi < bEnd; bi += bStep) {
+            batch(bi);
+            __syncthreads();  // the next batch overwrites the staged state and fluxes
+        }
+    } else {
+        if (bFirst < bEnd) batch(bFirst);
     }
     if ((a.stage == kStageRK4 || a.stage == kStageEuler) && a.snap) {
         // block reduction of the residual partial sums (deterministic order)
@@ -730,6 +764,304 @@ __global__ __launch_bounds__(kBlock) void stageKernelWave(
             c = cc;
         }
         waveSync();
+    }
+    if (isFinal && a.snap) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            double v = res[k];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0) sRed[wave][k] = v;
+        }
+        __syncthreads();
+        if (tid < 5) {
+            double sum = 0.0;
+            for (int w = 0; w < kBlock / 64; ++w) sum += sRed[w][tid];
+            a.resPartial[static_cast<int64_t>(blockIdx.x) * 5 + tid] = sum;
+        }
+    }
+}
+
+// --------------------------------------------------- block matrix-core stage kernel ----
+//
+// 16 elements per 256-thread block (4 waves x one 16-column tile of 4 elements x 4 fields):
+//  1. state, geometry and per-face data to LDS;
+//  2. VALU: the flux-Jacobian-folded volume operand G_r[j] = sum_x (dxi_r/dx_x) A_x q_j once
+//     per node, and the scaled numerical fluxes Fscale * pi once per face node, written as
+//     the B operand in matrix-core fragment order ([k-step][column tile][lane]; conflict-free);
+//  3. matrix core: rhs = [Dw_1 | .. | Dw_DIM | -LIFT] . B with the A fragments resident in
+//     registers, four independent accumulation chains per wave;
+//  4. the wave's result is transposed through its own (now dead) slice of B and the RK
+//     update is done node-major with 16-byte accesses, as in stageKernel.
+template <int DIM, int P>
+struct BlockMmShape {
+    using M = MfmaShape<DIM, P>;
+    static constexpr int EB = 16;                        // elements per block
+    static constexpr int KROWS = 4 * M::KS;              // B rows (volume, then lift)
+    static constexpr int kBFloats = M::KS * 4 * 64;      // B fragments of the 4 column tiles
+    static constexpr int kAPerLane = (M::MT * M::KS + 3) / 4 * 4;  // A values per lane (padded)
+    static constexpr bool kSupported = M::kSupported && Sizes<DIM, P>::NP <= KROWS;
+};
+
+template <int DIM, int P, typename T, bool COMP>
+__global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T* __restrict__ afrag) {
+    using S = Sizes<DIM, P>;
+    using L = OpLayout<DIM, P>;
+    using M = MfmaShape<DIM, P>;
+    using B = BlockMmShape<DIM, P>;
+    using MT_ = MfmaTraits<T>;
+    using V4 = typename Vec4<T>::V;
+    constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC, NFT = S::NFT, NF = 4, EB = B::EB;
+    constexpr int JJ = DIM * DIM;
+    constexpr int LIFT0 = 4 * M::KSV;  // first B row of the lift part
+
+    __shared__ __align__(16) T sB[B::kBFloats];
+    __shared__ __align__(16) T sQ[EB * NP * NF];
+    __shared__ T sJ[EB * JJ];
+    __shared__ __align__(16) T sN[EB * NFC * 4];
+    __shared__ int32_t sNb[EB * NFC];
+    __shared__ int32_t sCd[EB * NFC];
+    __shared__ int32_t sI[L::kITotal];
+    __shared__ double sRed[kBlock / 64][5];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t blk = xcdRemap(blockIdx.x, gridDim.x);
+    const int64_t e0 = blk * EB;
+    const int nE = static_cast<int>((a.K - e0) < EB ? (a.K - e0) : EB);
+    const int stage = a.stage;
+    const bool needU = stage == kStageRK2 || stage == kStageRK3 || stage == kStageRK4;
+    const bool isFinal = stage == kStageRK4 || stage == kStageEuler;
+
+    // A fragments of this lane (constant operators), resident for the whole block; stored
+    // lane-contiguous ([lane][mt][ks]) so that they load as 16-byte vectors
+    T af[B::kAPerLane];
+    {
+        const V4* src = reinterpret_cast<const V4*>(afrag + lane * B::kAPerLane);
+#pragma unroll
+        for (int x = 0; x < B::kAPerLane / 4; ++x) {
+            const V4 v = src[x];
+            af[4 * x] = v.x;
+            af[4 * x + 1] = v.y;
+            af[4 * x + 2] = v.z;
+            af[4 * x + 3] = v.w;
+        }
+    }
+
+    // 1. stage state, geometry, face data, index tables; zero the B padding rows
+    {
+        const V4* q4 = reinterpret_cast<const V4*>(a.qin + e0 * NP * NF);
+        for (int x = tid; x < nE * NP; x += kBlock) reinterpret_cast<V4*>(sQ)[x] = q4[x];
+        for (int x = tid; x < nE * JJ; x += kBlock) sJ[x] = a.geo[e0 * JJ + x];
+        for (int x = tid; x < nE * NFC * 4; x += kBlock) sN[x] = a.fgeo[e0 * NFC * 4 + x];
+        for (int x = tid; x < nE * NFC; x += kBlock) {
+            sNb[x] = a.nbr[e0 * NFC + x];
+            sCd[x] = a.code[e0 * NFC + x];
+        }
+        for (int x = tid; x < L::kITotal; x += kBlock) sI[x] = a.iops[x];
+        constexpr int padV = LIFT0 - M::KV, padL = B::KROWS - LIFT0 - NFT;
+        for (int x = tid; x < (padV + padL) * 64; x += kBlock) {
+            const int pr = x / 64, col = x - pr * 64;
+            const int kk = pr < padV ? M::KV + pr : LIFT0 + NFT + (pr - padV);
+            sB[((kk >> 2) * 4 + (col >> 4)) * 64 + (kk & 3) * 16 + (col & 15)] = T(0);
+        }
+    }
+    __syncthreads();
+
+    // B fragment index of (row kk, column 4*e + field 0); the 4 fields are contiguous
+    auto bIndex = [](int kk, int e) { return ((kk >> 2) * 4 + (e >> 2)) * 64 + (kk & 3) * 16 + (e & 3) * 4; };
+
+    // 2a. volume operand G_r[j] (one item per element node)
+    for (int it = tid; it < nE * NP; it += kBlock) {
+        const int e = it / NP, j = it - e * NP;
+        const V4 q = reinterpret_cast<const V4*>(sQ)[it];
+        const T* J = &sJ[e * JJ];
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) {
+            // sum_x J_rx F_x(q): F_x = (v0_x p + rho0 c0^2 v_x, v0_x v + e_x p / rho0)
+            T w = T(0), jv = T(0);
+            T jx[3] = {T(0), T(0), T(0)};
+#pragma unroll
+            for (int x = 0; x < DIM; ++x) {
+                jx[x] = J[r * DIM + x];
+                w += jx[x] * a.ph.v0[x];
+            }
+            jv = jx[0] * q.y;
+            if (DIM > 1) jv += jx[1] * q.z;
+            if (DIM > 2) jv += jx[DIM > 2 ? 2 : 0] * q.w;
+            V4 g;
+            g.x = w * q.x + a.ph.rc2 * jv;
+            g.y = w * q.y + jx[0] * q.x * a.ph.irho0;
+            g.z = w * q.z + jx[1] * q.x * a.ph.irho0;
+            g.w = w * q.w + jx[2] * q.x * a.ph.irho0;
+            reinterpret_cast<V4*>(sB)[bIndex(r * NP + j, e) >> 2] = g;
+        }
+    }
+    // 2b. scaled numerical fluxes (one item per element face node)
+    for (int it = tid; it < nE * NFT; it += kBlock) {
+        const int e = it / NFT, fa = it - e * NFT;
+        const int f = fa / NFP, an = fa - f * NFP;
+        const int ef = e * NFC + f;
+        const T nv[3] = {sN[ef * 4 + 0], sN[ef * 4 + 1], sN[ef * 4 + 2]};
+        const T fs = sN[ef * 4 + 3];
+        const int own = sI[L::kMask + f * NFP + an];
+        T qm[NF], qp[NF], pi[NF];
+        {
+            const V4 v = reinterpret_cast<const V4*>(sQ)[e * NP + own];
+            qm[0] = v.x; qm[1] = v.y; qm[2] = v.z; qm[3] = v.w;
+        }
+        const int nb = sNb[ef];
+        if (nb >= 0) {
+            const int cd = sCd[ef];
+            const int fp = cd & 3, pc = cd >> 2;
+            const int nn = sI[L::kMask + fp * NFP + sI[L::kPerm + pc * NFP + an]];
+            const V4 v = reinterpret_cast<const V4*>(a.qin)[static_cast<int64_t>(nb) * NP + nn];
+            qp[0] = v.x; qp[1] = v.y; qp[2] = v.z; qp[3] = v.w;
+        } else if (nb <= -3) {
+            const V4 v = reinterpret_cast<const V4*>(a.ghost)[static_cast<int64_t>(-3 - nb) * NFP + an];
+            qp[0] = v.x; qp[1] = v.y; qp[2] = v.z; qp[3] = v.w;
+        } else {
+            qp[0] = qp[1] = qp[2] = qp[3] = T(0);
+        }
+        numericalFlux<DIM, NF>(a.ph, nv, nb <= -3 ? 0 : nb, qm, qp, pi);
+        V4 out;
+        out.x = fs * pi[0]; out.y = fs * pi[1]; out.z = fs * pi[2]; out.w = fs * pi[3];
+        reinterpret_cast<V4*>(sB)[bIndex(LIFT0 + fa, e) >> 2] = out;
+    }
+    constexpr int NIT = (4 * NP + 63) / 64;  // update items per lane (it = lane + 64 c)
+    const int nEw = nE - 4 * wave;           // elements of this wave present in the block
+    __syncthreads();
+
+    // 3. matrix core: this wave's column tile (elements 4*wave .. 4*wave+3)
+    typename MT_::V acc[M::MT][2];
+#pragma unroll
+    for (int mt = 0; mt < M::MT; ++mt) acc[mt][0] = acc[mt][1] = typename MT_::V{T(0), T(0), T(0), T(0)};
+    T* Bw = sB + wave * 64 + lane;
+#pragma unroll
+    for (int ks = 0; ks < M::KS; ++ks) {
+        const T b = Bw[ks * 256];
+#pragma unroll
+        for (int mt = 0; mt < M::MT; ++mt) acc[mt][ks & 1] = MT_::mma(af[mt * M::KS + ks], b, acc[mt][ks & 1]);
+    }
+    waveSync();
+    // 4a. transpose through this wave's own slice of B: element row i, column lane%16
+#pragma unroll
+    for (int mt = 0; mt < M::MT; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * mt + MT_::row(lane, r) - 0;
+            if (row < NP) {
+                const T v = acc[mt][0][r] + acc[mt][1][r];
+                Bw[((row >> 2) * 4) * 64 + (row & 3) * 16 - lane + (lane & 15)] = v;
+            }
+        }
+    waveSync();
+
+    // 4b. RK update, node-major over this wave's 4 elements
+    double res[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < NIT; ++c) {
+        const int it = lane + 64 * c;
+        const int el = it / NP, i = it - el * NP;
+        if (it >= 4 * NP || el >= nEw) continue;
+        const int eb = 4 * wave + el;
+        const V4 rv = reinterpret_cast<const V4*>(sB)[(((i >> 2) * 4 + wave) * 64 + (i & 3) * 16 + el * 4) >> 2];
+        const T rhs[4] = {rv.x, rv.y, rv.z, rv.w};
+        const int64_t node = (e0 + eb) * NP + i;
+        const V4 q0v = reinterpret_cast<const V4*>(sQ)[eb * NP + i];
+        const T q0[4] = {q0v.x, q0v.y, q0v.z, q0v.w};
+        T pu[4] = {T(0), T(0), T(0), T(0)}, pacc[4] = {T(0), T(0), T(0), T(0)}, plo[4] = {T(0), T(0), T(0), T(0)};
+        if (needU) {
+            const V4 x = reinterpret_cast<const V4*>(a.u)[node];
+            const V4 y = reinterpret_cast<const V4*>(a.acc)[node];
+            pu[0] = x.x; pu[1] = x.y; pu[2] = x.z; pu[3] = x.w;
+            pacc[0] = y.x; pacc[1] = y.y; pacc[2] = y.z; pacc[3] = y.w;
+        }
+        if constexpr (COMP) {
+            const V4 x = reinterpret_cast<const V4*>(a.ulo)[node];
+            plo[0] = x.x; plo[1] = x.y; plo[2] = x.z; plo[3] = x.w;
+        }
+        const T dt = a.dt;
+        if (stage == kStageRHS) {
+            V4 o;
+            o.x = dt * rhs[0]; o.y = dt * rhs[1]; o.z = dt * rhs[2]; o.w = dt * rhs[3];
+            reinterpret_cast<V4*>(a.qout)[node] = o;
+        } else if (stage <= kStageRK3) {
+            const T half = (stage == kStageRK3) ? T(1) : T(0.5);
+            T accn[4], qn[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const T kk = dt * rhs[k];
+                T uv;
+                if (stage == kStageRK1) {
+                    uv = q0[k];
+                    accn[k] = kk;
+                } else {
+                    uv = pu[k];
+                    accn[k] = pacc[k] + T(2) * kk;
+                }
+                if constexpr (COMP)
+                    qn[k] = uv + (plo[k] + half * kk);
+                else
+                    qn[k] = uv + half * kk;
+            }
+            V4 oa, oq;
+            oa.x = accn[0]; oa.y = accn[1]; oa.z = accn[2]; oa.w = accn[3];
+            oq.x = qn[0]; oq.y = qn[1]; oq.z = qn[2]; oq.w = qn[3];
+            reinterpret_cast<V4*>(a.acc)[node] = oa;
+            reinterpret_cast<V4*>(a.qout)[node] = oq;
+        } else {
+            T unew[4], lw[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const T kk = dt * rhs[k];
+                const T delta = (stage == kStageRK4) ? (pacc[k] + kk) / T(6) : kk;
+                const T uv = (stage == kStageRK4) ? pu[k] : q0[k];
+                if constexpr (COMP) {
+                    T s_, e_, t_, e2;
+                    twoSum(plo[k], delta, t_, e2);
+                    twoSum(uv, t_, s_, e_);
+                    const T nlo = e_ + e2;
+                    T h, l2;
+                    twoSum(s_, nlo, h, l2);
+                    unew[k] = h;
+                    lw[k] = l2;
+                } else {
+                    unew[k] = uv + delta;
+                    lw[k] = T(0);
+                }
+            }
+            V4 oh;
+            oh.x = unew[0]; oh.y = unew[1]; oh.z = unew[2]; oh.w = unew[3];
+            reinterpret_cast<V4*>(a.uout)[node] = oh;
+            if constexpr (COMP) {
+                V4 ol;
+                ol.x = lw[0]; ol.y = lw[1]; ol.z = lw[2]; ol.w = lw[3];
+                reinterpret_cast<V4*>(a.ulo)[node] = ol;
+            }
+            if (a.snap) {
+                const V4 sv4 = reinterpret_cast<const V4*>(a.snap)[node];
+                double sv[4] = {static_cast<double>(sv4.x), static_cast<double>(sv4.y), static_cast<double>(sv4.z),
+                                static_cast<double>(sv4.w)};
+                if constexpr (COMP) {
+                    const V4 sl = reinterpret_cast<const V4*>(a.snaplo)[node];
+                    sv[0] += static_cast<double>(sl.x);
+                    sv[1] += static_cast<double>(sl.y);
+                    sv[2] += static_cast<double>(sl.z);
+                    sv[3] += static_cast<double>(sl.w);
+                }
+                double d[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    d[k] = (static_cast<double>(unew[k]) + static_cast<double>(lw[k])) - sv[k];
+                const double ic2 = static_cast<double>(a.ph.ic2);
+                const double u0 = static_cast<double>(unew[0]) + static_cast<double>(lw[0]);
+                const double sp = sv[0] * ic2 - u0 * ic2;
+                res[0] += d[0] * d[0];
+                res[1] += sp * sp;
+                res[2] += d[1] * d[1];
+                res[3] += d[2] * d[2];
+                res[4] += d[3] * d[3];
+            }
+        }
     }
     if (isFinal && a.snap) {
 #pragma unroll

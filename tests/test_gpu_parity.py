@@ -29,7 +29,7 @@ RHO0 = 1.225
     (3, 2, 4, [(0x10, 1), (0x2F, 0)], (30, 0, 0), 340.0),
     (3, 1, 5, [(ALL3, 0)], (30, 0, 0), 340.0),
 ])
-@pytest.mark.parametrize("kernel", ["valu", "wave"])
+@pytest.mark.parametrize("kernel", ["valu", "mm", "wave"])
 def test_rhs_matches_oracle(dim, n, p, groups, v0, c0, kernel, monkeypatch):
     print(f"case rhs dim={dim} n={n} p={p} kernel={kernel}", flush=True)
     monkeypatch.setenv("DGFEM_KERNEL", kernel)
@@ -72,7 +72,7 @@ def test_rk4_2d_reflecting_matches_oracle():
     np.testing.assert_allclose(r.observers, ref["observers"], rtol=1e-8, atol=1e-12)
 
 
-@pytest.mark.parametrize("kernel", ["valu", "wave"])
+@pytest.mark.parametrize("kernel", ["valu", "mm", "wave"])
 def test_rk4_3d_absorbing_monopole_matches_oracle(kernel, monkeypatch):
     """Reduced 3D case: 4^3 x 6 jittered Kuhn tets, p=3, v0=(30,0,0), c0=340, all
     absorbing, monopole at the origin."""
@@ -133,7 +133,7 @@ def test_graph_replay_equals_eager():
     assert np.array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("kernel", ["valu", "wave"])
+@pytest.mark.parametrize("kernel", ["valu", "mm", "wave"])
 def test_mixed_precision_close_to_fp64(kernel, monkeypatch):
     """Mixed precision (FP32 state / fluxes, compensated RK4 accumulation) against GPU FP64,
     3D absorbing case with a monopole, 500 steps: pressure rel L2 <= 1e-6."""
