@@ -140,6 +140,47 @@ __device__ inline void numericalFlux(const Physics<T>& ph, const T n[3], int nb,
 }
 
 template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+    using V = float4;
+};
+template <>
+struct Vec4<double> {
+    using V = double4;
+};
+
+// One node's field vector: a single 16-byte (FP32) / 2x16-byte (FP64) access for 4 fields.
+template <int NF, typename T>
+__device__ inline void loadNode(const T* __restrict__ p, T (&v)[NF]) {
+    if constexpr (NF == 4) {
+        const typename Vec4<T>::V x = *reinterpret_cast<const typename Vec4<T>::V*>(p);
+        v[0] = x.x;
+        v[1] = x.y;
+        v[2] = x.z;
+        v[3] = x.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) v[k] = p[k];
+    }
+}
+
+template <int NF, typename T>
+__device__ inline void storeNode(T* __restrict__ p, const T (&v)[NF]) {
+    if constexpr (NF == 4) {
+        typename Vec4<T>::V x;
+        x.x = v[0];
+        x.y = v[1];
+        x.z = v[2];
+        x.w = v[3];
+        *reinterpret_cast<typename Vec4<T>::V*>(p) = x;
+    } else {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) p[k] = v[k];
+    }
+}
+
+template <typename T>
 __device__ inline void twoSum(T a, T b, T& s, T& e) {
     s = a + b;
     T bb = s - a;
@@ -171,7 +212,7 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
     // host sizes the grid accordingly.
     constexpr bool kPersistent = sizeof(T) == 8;
 
-    __shared__ T sQ[EB * NP * NF];
+    __shared__ __align__(16) T sQ[EB * NP * NF];
     __shared__ T sF[EB * NFT * NF];
     __shared__ T sJ[EB * JJ];
     __shared__ T sN[kFaceLds * 4];       // per (element, face): outward unit normal, Fscale
@@ -211,23 +252,27 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
     const int64_t myBase = (e0 * NP + tid) * NF;
     T pu[NF], pacc[NF], plo[NF], ps[NF], psl[NF];
 #pragma unroll
-    for (int k = 0; k < NF; ++k) {
-        pu[k] = pacc[k] = plo[k] = ps[k] = psl[k] = T(0);
-        if (active) {
-            if (needU) {
-                pu[k] = a.u[myBase + k];
-                pacc[k] = a.acc[myBase + k];
-            }
-            if constexpr (COMP) plo[k] = a.ulo[myBase + k];
-            if (isFinal && a.snap) {
-                ps[k] = a.snap[myBase + k];
-                if constexpr (COMP) psl[k] = a.snaplo[myBase + k];
-            }
+    for (int k = 0; k < NF; ++k) pu[k] = pacc[k] = plo[k] = ps[k] = psl[k] = T(0);
+    if (active) {
+        if (needU) {
+            loadNode<NF>(a.u + myBase, pu);
+            loadNode<NF>(a.acc + myBase, pacc);
+        }
+        if constexpr (COMP) loadNode<NF>(a.ulo + myBase, plo);
+        if (isFinal && a.snap) {
+            loadNode<NF>(a.snap + myBase, ps);
+            if constexpr (COMP) loadNode<NF>(a.snaplo + myBase, psl);
         }
     }
     {
         const T* src = a.qin + e0 * NP * NF;
-        for (int x = tid; x < nE * NP * NF; x += kBlock) sQ[x] = src[x];
+        if constexpr (NF == 4) {
+            using V4 = typename Vec4<T>::V;
+            for (int x = tid; x < nE * NP; x += kBlock)
+                reinterpret_cast<V4*>(sQ)[x] = reinterpret_cast<const V4*>(src)[x];
+        } else {
+            for (int x = tid; x < nE * NP * NF; x += kBlock) sQ[x] = src[x];
+        }
         const T* g = a.geo + e0 * JJ;
         for (int x = tid; x < nE * JJ; x += kBlock) sJ[x] = g[x];
     }
@@ -344,29 +389,34 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
         const int64_t base = node * NF;
         const T dt = a.dt;
         if (a.stage == kStageRHS) {
+            T o[NF];
 #pragma unroll
-            for (int k = 0; k < NF; ++k) a.qout[base + k] = dt * rhs[k];
+            for (int k = 0; k < NF; ++k) o[k] = dt * rhs[k];
+            storeNode<NF>(a.qout + base, o);
         } else if (a.stage <= kStageRK3) {
             const T half = (a.stage == kStageRK3) ? T(1) : T(0.5);
+            T accn[NF], qn[NF];
 #pragma unroll
             for (int k = 0; k < NF; ++k) {
                 const T kk = dt * rhs[k];
                 T uv;
                 if (a.stage == kStageRK1) {
                     uv = q0[i * NF + k];
-                    a.acc[base + k] = kk;
+                    accn[k] = kk;
                 } else {
                     uv = pu[k];
-                    a.acc[base + k] = pacc[k] + T(2) * kk;
+                    accn[k] = pacc[k] + T(2) * kk;
                 }
                 if constexpr (COMP)
-                    a.qout[base + k] = uv + (plo[k] + half * kk);
+                    qn[k] = uv + (plo[k] + half * kk);
                 else
-                    a.qout[base + k] = uv + half * kk;
+                    qn[k] = uv + half * kk;
             }
+            storeNode<NF>(a.acc + base, accn);
+            storeNode<NF>(a.qout + base, qn);
         } else {
             // final update: u += (k1 + 2k2 + 2k3 + k4)/6  (RK4)  or  u += k  (Euler)
-            T unew[NF];
+            T unew[NF], lnew[NF];
             double lo[NF];
 #pragma unroll
             for (int k = 0; k < NF; ++k) {
@@ -380,17 +430,16 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
                     const T nlo = e + e2;
                     T h, l2;
                     twoSum(s, nlo, h, l2);
-                    a.uout[base + k] = h;
-                    a.ulo[base + k] = l2;
                     unew[k] = h;
+                    lnew[k] = l2;
                     lo[k] = static_cast<double>(l2);
                 } else {
-                    const T un = uv + delta;
-                    a.uout[base + k] = un;
-                    unew[k] = un;
+                    unew[k] = uv + delta;
                     lo[k] = 0.0;
                 }
             }
+            storeNode<NF>(a.uout + base, unew);
+            if constexpr (COMP) storeNode<NF>(a.ulo + base, lnew);
             if (a.snap) {
                 double d[4];
                 double sv[4];
@@ -491,16 +540,6 @@ __device__ inline void waveSync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-template <typename T>
-struct Vec4;
-template <>
-struct Vec4<float> {
-    using V = float4;
-};
-template <>
-struct Vec4<double> {
-    using V = double4;
-};
 
 template <int DIM, int P>
 struct WaveShape {
