@@ -215,9 +215,9 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
     __shared__ __align__(16) T sQ[EB * NP * NF];
     __shared__ T sF[EB * NFT * NF];
     __shared__ T sJ[EB * JJ];
-    __shared__ T sN[kFaceLds * 4];       // per (element, face): outward unit normal, Fscale
-    __shared__ int32_t sNb[kFaceLds];    // per (element, face): neighbour / boundary code
-    __shared__ int32_t sCd[kFaceLds];    // per (element, face): neighbour face | perm << 2
+    __shared__ __align__(16) T sN[kFaceLds * 4];     // per (element, face): unit normal, Fscale
+    __shared__ __align__(16) int32_t sNb[kFaceLds];  // per (element, face): neighbour / boundary code
+    __shared__ __align__(16) uint8_t sCd8[kFaceLds]; // per (element, face): neighbour face | perm << 2
     __shared__ T sOps[kOpsLds];
     __shared__ int32_t sI[L::kITotal];
     __shared__ double sRed[kBlock / 64][5];
@@ -279,11 +279,20 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
     // per (element, face): unit normal, Fscale, neighbour and code, staged once for all the
     // face nodes of the block
     if constexpr (kStageFaces) {
-        const T* fg = a.fgeo + e0 * NFC * 4;
-        for (int x = tid; x < nE * NFC * 4; x += kBlock) sN[x] = fg[x];
-        for (int x = tid; x < nE * NFC; x += kBlock) {
-            sNb[x] = a.nbr[e0 * NFC + x];
-            sCd[x] = a.code[e0 * NFC + x];
+        // rows of 4 (normal + Fscale; 4 neighbours / codes of one element for tets) as vectors
+        using V4 = typename Vec4<T>::V;
+        const V4* fg = reinterpret_cast<const V4*>(a.fgeo + e0 * NFC * 4);
+        for (int x = tid; x < nE * NFC; x += kBlock) reinterpret_cast<V4*>(sN)[x] = fg[x];
+        if constexpr (NFC == 4) {
+            for (int x = tid; x < nE; x += kBlock) {
+                reinterpret_cast<int4*>(sNb)[x] = reinterpret_cast<const int4*>(a.nbr + e0 * NFC)[x];
+                reinterpret_cast<uchar4*>(sCd8)[x] = reinterpret_cast<const uchar4*>(a.code + e0 * NFC)[x];
+            }
+        } else {
+            for (int x = tid; x < nE * NFC; x += kBlock) {
+                sNb[x] = a.nbr[e0 * NFC + x];
+                sCd8[x] = a.code[e0 * NFC + x];
+            }
         }
     }
     __syncthreads();
@@ -299,20 +308,15 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
         const T fs = nf[3];
         const int own = sI[L::kMask + f * NFP + an];
         T qm[NF], qp[NF], pi[NF];
-#pragma unroll
-        for (int k = 0; k < NF; ++k) qm[k] = sQ[(el * NP + own) * NF + k];
+        loadNode<NF>(&sQ[(el * NP + own) * NF], qm);
         const int nb = kStageFaces ? sNb[efl] : a.nbr[efg];
         if (nb >= 0) {
-            const int cd = kStageFaces ? sCd[efl] : a.code[efg];
+            const int cd = kStageFaces ? sCd8[efl] : a.code[efg];
             const int fp = cd & 3, pc = cd >> 2;
             const int nn = sI[L::kMask + fp * NFP + sI[L::kPerm + pc * NFP + an]];
-            const T* src = a.qin + (static_cast<int64_t>(nb) * NP + nn) * NF;
-#pragma unroll
-            for (int k = 0; k < NF; ++k) qp[k] = src[k];
+            loadNode<NF>(a.qin + (static_cast<int64_t>(nb) * NP + nn) * NF, qp);
         } else if (nb <= -3) {
-            const T* src = a.ghost + (static_cast<int64_t>(-3 - nb) * NFP + an) * NF;
-#pragma unroll
-            for (int k = 0; k < NF; ++k) qp[k] = src[k];
+            loadNode<NF>(a.ghost + (static_cast<int64_t>(-3 - nb) * NFP + an) * NF, qp);
         } else {
 #pragma unroll
             for (int k = 0; k < NF; ++k) qp[k] = T(0);
