@@ -165,6 +165,35 @@ __device__ inline void loadNode(const T* __restrict__ p, T (&v)[NF]) {
     }
 }
 
+// Streamed once per stage (base state, accumulator, compensation, snapshot): non-temporal, so
+// that the caches keep the stage input that neighbour gathers re-read.
+template <int NF, typename T>
+__device__ inline void loadNodeNT(const T* __restrict__ p, T (&v)[NF]) {
+    if constexpr (NF == 4) {
+        typedef T V __attribute__((ext_vector_type(4)));
+        const V x = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+        v[0] = x[0];
+        v[1] = x[1];
+        v[2] = x[2];
+        v[3] = x[3];
+    } else {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) v[k] = __builtin_nontemporal_load(p + k);
+    }
+}
+
+template <int NF, typename T>
+__device__ inline void storeNodeNT(T* __restrict__ p, const T (&v)[NF]) {
+    if constexpr (NF == 4) {
+        typedef T V __attribute__((ext_vector_type(4)));
+        const V x = {v[0], v[1], v[2], v[3]};
+        __builtin_nontemporal_store(x, reinterpret_cast<V*>(p));
+    } else {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) __builtin_nontemporal_store(v[k], p + k);
+    }
+}
+
 template <int NF, typename T>
 __device__ inline void storeNode(T* __restrict__ p, const T (&v)[NF]) {
     if constexpr (NF == 4) {
@@ -255,13 +284,13 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
     for (int k = 0; k < NF; ++k) pu[k] = pacc[k] = plo[k] = ps[k] = psl[k] = T(0);
     if (active) {
         if (needU) {
-            loadNode<NF>(a.u + myBase, pu);
-            loadNode<NF>(a.acc + myBase, pacc);
+            loadNodeNT<NF>(a.u + myBase, pu);
+            loadNodeNT<NF>(a.acc + myBase, pacc);
         }
-        if constexpr (COMP) loadNode<NF>(a.ulo + myBase, plo);
+        if constexpr (COMP) loadNodeNT<NF>(a.ulo + myBase, plo);
         if (isFinal && a.snap) {
-            loadNode<NF>(a.snap + myBase, ps);
-            if constexpr (COMP) loadNode<NF>(a.snaplo + myBase, psl);
+            loadNodeNT<NF>(a.snap + myBase, ps);
+            if constexpr (COMP) loadNodeNT<NF>(a.snaplo + myBase, psl);
         }
     }
     {
@@ -416,7 +445,7 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
                 else
                     qn[k] = uv + half * kk;
             }
-            storeNode<NF>(a.acc + base, accn);
+            storeNodeNT<NF>(a.acc + base, accn);
             storeNode<NF>(a.qout + base, qn);
         } else {
             // final update: u += (k1 + 2k2 + 2k3 + k4)/6  (RK4)  or  u += k  (Euler)
@@ -442,8 +471,8 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
                     lo[k] = 0.0;
                 }
             }
-            storeNode<NF>(a.uout + base, unew);
-            if constexpr (COMP) storeNode<NF>(a.ulo + base, lnew);
+            storeNodeNT<NF>(a.uout + base, unew);
+            if constexpr (COMP) storeNodeNT<NF>(a.ulo + base, lnew);
             if (a.snap) {
                 double d[4];
                 double sv[4];
