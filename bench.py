@@ -36,7 +36,7 @@ BYTES_PER_DOF = {"fp64": 212.0, "mixed": 115.0}
 
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH/WRITE summary."""
-    path = os.path.join(ROOT, "profiles", "stage_kernel_c2_traffic.md")
+    path = os.path.join(ROOT, "profiles", "stage_kernel_c2_traffic.txt")
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch " + kernel + ":"):
