@@ -287,7 +287,10 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
             loadNodeNT<NF>(a.u + myBase, pu);
             loadNodeNT<NF>(a.acc + myBase, pacc);
         }
-        if constexpr (COMP) loadNodeNT<NF>(a.ulo + myBase, plo);
+        // the compensation word enters only the final accumulation: stage inputs are the FP32
+        // u + c k (measured: same error against FP64 after 2000 steps, 4 B/DOF less per stage)
+        if constexpr (COMP)
+            if (isFinal) loadNodeNT<NF>(a.ulo + myBase, plo);
         if (isFinal && a.snap) {
             loadNodeNT<NF>(a.snap + myBase, ps);
             if constexpr (COMP) loadNodeNT<NF>(a.snaplo + myBase, psl);
@@ -440,10 +443,7 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
                     uv = pu[k];
                     accn[k] = pacc[k] + T(2) * kk;
                 }
-                if constexpr (COMP)
-                    qn[k] = uv + (plo[k] + half * kk);
-                else
-                    qn[k] = uv + half * kk;
+                qn[k] = uv + half * kk;
             }
             storeNodeNT<NF>(a.acc + base, accn);
             storeNode<NF>(a.qout + base, qn);
@@ -758,7 +758,9 @@ __global__ __launch_bounds__(kBlock) void stageKernelWave(
                 pu[0] = reinterpret_cast<const V4*>(a.u)[off];
                 pacc[0] = reinterpret_cast<const V4*>(a.acc)[off];
             }
-            if constexpr (COMP) plo[0] = reinterpret_cast<const V4*>(a.ulo)[off];
+            plo[0] = V4{T(0), T(0), T(0), T(0)};
+            if constexpr (COMP)
+                if (isFinal) plo[0] = reinterpret_cast<const V4*>(a.ulo)[off];
             if (isFinal && a.snap) {
                 ps[0] = reinterpret_cast<const V4*>(a.snap)[off];
                 if constexpr (COMP) psl[0] = reinterpret_cast<const V4*>(a.snaplo)[off];
@@ -776,10 +778,7 @@ __global__ __launch_bounds__(kBlock) void stageKernelWave(
                 for (int k = 0; k < 4; ++k) {
                     const T uv = (stage == kStageRK1) ? qv[k] : uv4[k];
                     na[k] = (stage == kStageRK1) ? kk[k] : ac4[k] + T(2) * kk[k];
-                    if constexpr (COMP)
-                        nq[k] = uv + (lo4[k] + half * kk[k]);
-                    else
-                        nq[k] = uv + half * kk[k];
+                    nq[k] = uv + half * kk[k];
                 }
                 V4 oa, oq;
                 oa.x = na[0]; oa.y = na[1]; oa.z = na[2]; oa.w = na[3];
@@ -1048,8 +1047,10 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
             pacc[0] = y.x; pacc[1] = y.y; pacc[2] = y.z; pacc[3] = y.w;
         }
         if constexpr (COMP) {
-            const V4 x = reinterpret_cast<const V4*>(a.ulo)[node];
-            plo[0] = x.x; plo[1] = x.y; plo[2] = x.z; plo[3] = x.w;
+            if (isFinal) {
+                const V4 x = reinterpret_cast<const V4*>(a.ulo)[node];
+                plo[0] = x.x; plo[1] = x.y; plo[2] = x.z; plo[3] = x.w;
+            }
         }
         const T dt = a.dt;
         if (stage == kStageRHS) {
@@ -1070,10 +1071,7 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
                     uv = pu[k];
                     accn[k] = pacc[k] + T(2) * kk;
                 }
-                if constexpr (COMP)
-                    qn[k] = uv + (plo[k] + half * kk);
-                else
-                    qn[k] = uv + half * kk;
+                qn[k] = uv + half * kk;
             }
             V4 oa, oq;
             oa.x = accn[0]; oa.y = accn[1]; oa.z = accn[2]; oa.w = accn[3];
