@@ -1,0 +1,103 @@
+"""Throughput and accuracy of the GPU path on the named synthetic workloads (FP64 and mixed
+precision from the same initial state; mixed-precision error reported against FP64).
+
+  dense3d: 3D LEE, [-1,1]^3 as 44^3 hexes x 6 Kuhn tets (511,104 tets), interior vertices
+           jittered by 0.15h, p=5 (56 nodes), 5 fields (p, v, rho), v0=(30,0,0), rho0=1.225,
+           c0=340; sum of 8 seeded Gaussian pulses (centres U[-0.6,0.6]^3, widths U[0.05,0.15]
+           used as exp(-r^2 / width^2), amplitudes N(0,1)); reflecting at z=-1 (group 2),
+           absorbing elsewhere (group 1); CFL-limited dt.
+  lee2d:   the 2D LEE case on [-1,1]^2 (96x96x2 triangles, p=3, v0=(-30,30), c0=100,
+           reflecting, Gaussian pulse exp(-r^2/0.1)), CFL-limited dt.
+
+    python dgfem-acoustic_amd/workloads.py --workload dense3d --steps 200
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import numpy as np
+
+import dgfem_amd as dg
+
+RHO0 = 1.225
+
+
+def dense3d(n=44, seed=3):
+    m = dg.box_mesh(3, n, 5, jitter=0.15, seed=seed)
+    # group 1 (absorbing): all sides but z-; group 2 (reflecting): z-  (ascending tag order)
+    bc = dg.resolve_bc_sides(m, [(0x2F, dg.DG_BC_ABSORBING), (0x10, dg.DG_BC_REFLECTING)])
+    rng = np.random.default_rng(seed)
+    X = m.node_coords.reshape(-1, 3)
+    p = np.zeros(len(X))
+    for _ in range(8):
+        c = rng.uniform(-0.6, 0.6, 3)
+        w = rng.uniform(0.05, 0.15)
+        a = rng.normal(0.0, 1.0)
+        p += a * np.exp(-((X - c) ** 2).sum(axis=1) / (w * w))
+    c0 = 340.0
+    u = np.zeros((5, len(X)))
+    u[0] = p
+    u[4] = p / (c0 * c0)
+    return dict(mesh=m, bc=bc, order=5, v0=(30.0, 0.0, 0.0), c0=c0, fields=dg.DG_FIELDS_LEE5, u0=u,
+                describe=f"3D LEE, {n}^3x6 jittered Kuhn tets (K={m.K}), p=5, 5 fields, 8 Gaussian pulses, "
+                         "reflecting z=-1 + absorbing, CFL-limited dt")
+
+
+def lee2d(n=96, seed=1):
+    m = dg.box_mesh(2, n, 3, jitter=0.2, seed=seed)
+    bc = dg.resolve_bc_sides(m, [(0xF, dg.DG_BC_REFLECTING)])
+    X = m.node_coords.reshape(-1, 3)
+    u = np.zeros((4, len(X)))
+    u[0] = np.exp(-(X ** 2).sum(axis=1) / 0.1)
+    return dict(mesh=m, bc=bc, order=3, v0=(-30.0, 30.0, 0.0), c0=100.0, fields=dg.DG_FIELDS_REF4, u0=u,
+                describe=f"2D LEE, {n}^2x2 jittered triangles (K={m.K}), p=3, reflecting, Gaussian pulse, "
+                         "CFL-limited dt")
+
+
+WORKLOADS = {"dense3d": dense3d, "lee2d": lee2d}
+
+
+def run(name, steps, warmup=10):
+    w = WORKLOADS[name]()
+    m = w["mesh"]
+    out = dict(workload=name, describe=w["describe"], steps=steps)
+    states = {}
+    for label, prec in (("fp64", dg.DG_FP64), ("mixed", dg.DG_MIXED)):
+        s = dg.Solver(m.node_coords, m.element_vertices, w["order"], w["bc"], v0=w["v0"], rho0=RHO0, c0=w["c0"],
+                      fields=w["fields"], precision=prec, steps_per_graph=20)
+        info = s.info()
+        dt = info.min_inradius / ((w["c0"] + np.linalg.norm(w["v0"])) * (2 * w["order"] + 1))
+        s.set_state(w["u0"])
+        s.run_steps(warmup, dt)           # graph capture and warm-up
+        s.set_state(w["u0"])
+        t0 = time.perf_counter()
+        r = s.run_steps(steps, dt)
+        wall = time.perf_counter() - t0
+        dofs = info.num_elements * info.nodes_per_element * s.nf
+        stage, _, post = s.profile_steps(dt, 5)
+        states[label] = (s.get_state(), r.observers)
+        out[label] = dict(dof_updates_per_s=dofs * steps / (r.device_ms * 1e-3), ms_per_step=r.device_ms / steps,
+                          wall_ms_per_step=1e3 * wall / steps, stage_ms=[float(v) for v in stage],
+                          post_ms=float(post), dt=dt, dofs=int(dofs), device_bytes=int(info.device_bytes))
+        s.close()
+        print(f"# {label}: {out[label]['dof_updates_per_s']:.4e} DOF-updates/s, {out[label]['ms_per_step']:.4f} "
+              f"ms/step", flush=True)
+    up, _ = states["fp64"]
+    um, _ = states["mixed"]
+    out["rel_l2_pressure_mixed_vs_fp64"] = float(np.linalg.norm(um[0] - up[0]) / np.linalg.norm(up[0]))
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["dof_updates_per_s"] / out["fp64"]["dof_updates_per_s"]
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="dense3d")
+    ap.add_argument("--steps", type=int, default=200)
+    a = ap.parse_args(argv)
+    print(json.dumps(run(a.workload, a.steps)), flush=True)
+
+
+if __name__ == "__main__":
+    main()
