@@ -36,6 +36,9 @@ DATA_FILES = [
     "cases/2d_hilltop/2d_hilltop.conf",
     "cases/2d_hilltop/2d_hilltop_fs.msh",
     "doc/2d/square_reflection.msh",
+    "doc/2d/square2.msh",
+    "doc/2d/room.msh",
+    "doc/3d/sphere.msh",
     "data/data.csv",
     "data/data1.wav",
     "residuals.csv",

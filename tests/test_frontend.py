@@ -189,3 +189,67 @@ def test_missing_files_report_reference_messages(tmp_path, monkeypatch):
     with pytest.raises(dg.DGError) as e:
         dg.case_summary("m.conf")
     assert "Mesh file 'missing.msh' not found or corrupted" in str(e.value)
+
+
+MESH_CONF = """[Config]
+meshFileName = {mesh}
+timeStart=0
+timeEnd=0.01
+timeStep=1e-4
+timeRate=0.005
+elementType=Lagrange
+timeIntMethod=Runge-Kutta
+{bcs}
+numThreads=1
+v0_x = 0
+v0_y = 0
+v0_z = 0
+rho0 = 1.225
+c0 = 340
+"""
+
+
+def _mesh_summary(tmp_path, monkeypatch, rel, bcs):
+    _stage_case(tmp_path, rel)
+    (tmp_path / "m.conf").write_text(MESH_CONF.format(mesh=rel, bcs="\n".join(bcs)))
+    monkeypatch.chdir(tmp_path)
+    return dg.case_summary("m.conf")
+
+
+@pytest.mark.parametrize("rel,tag_types,order,npe", [
+    ("doc/2d/square2.msh", {9: 0}, 2, 6),     # second-order triangles, physical 'Abs'
+    ("doc/3d/sphere.msh", {1: 0}, 1, 4),      # linear tetrahedra, physical surface 'Absorbing'
+])
+def test_reference_meshes_against_python_reader(tmp_path, monkeypatch, rel, tag_types, order, npe):
+    """Element count, order, node count per element and boundary faces of the reference's own
+    p=2 triangle mesh and tetrahedral mesh, against the independent Python reader and the
+    first-node rule (src/Mesh.cpp:579-605)."""
+    from msh_py import first_node_bc_any, read_msh41
+    m = read_msh41(os.path.join(CASES, rel))
+    c = _mesh_summary(tmp_path, monkeypatch, rel, ["Abs = Absorbing", "Absorbing = Absorbing"])
+    assert c["dim"] == m["dim"] and c["order"] == order and c["nodes_per_element"] == npe
+    assert c["elements"] == len(m["tags"]) and m["tags"].shape[1] == npe
+    bc = first_node_bc_any(m["verts"], m["dim"], m["groups"], tag_types)
+    assert c["bc_counts"]["absorbing"] == sum(1 for v in bc.values() if v == 0)
+    assert c["bc_counts"]["reflecting"] == sum(1 for v in bc.values() if v == 1) == 0
+    assert [n for _, n in c["physical_bcs"]] == ["Absorbing"]
+    # a closed boundary: every boundary face is typed and the count is exactly the unpaired faces
+    assert len(bc) == sum(c["bc_counts"].values())
+
+
+def test_reference_msh40_mesh(tmp_path, monkeypatch):
+    """The reference's room mesh is written in MSH 4.0 (header '4 0 8'): the reader takes it
+    and its triangle count matches the file's element blocks."""
+    rel = "doc/2d/room.msh"
+    L = open(os.path.join(CASES, rel)).read().split("\n")
+    assert L[1].split()[:2] == ["4", "0"]
+    i = L.index("$Elements")
+    j, ntri = i + 2, 0
+    for _ in range(int(L[i + 1].split()[0])):
+        a = list(map(int, L[j].split()))   # 4.0 block header: entity tag, dim, type, count
+        if a[1] == 2 and a[2] == 2:
+            ntri += a[3]
+        j += 1 + a[3]
+    c = _mesh_summary(tmp_path, monkeypatch, rel, ["Absorbing = Absorbing"])
+    assert c["dim"] == 2 and c["order"] == 1 and c["elements"] == ntri == 5199
+    assert c["bc_counts"] == {"absorbing": 451, "reflecting": 0}
