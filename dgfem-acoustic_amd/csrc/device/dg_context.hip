@@ -146,6 +146,7 @@ dgk::Physics<T> makePhysics(const dg_physics& ph) {
 
 template <int DIM, int P, int NF, typename T, bool COMP>
 void launchStage(dg_context* c, int stage, const void* qin, void* qout, void* uout, bool snap) {
+    using dgk::stageKernel;
     dgk::StageArgs<T> a{};
     a.qin = static_cast<const T*>(qin);
     a.qout = static_cast<T*>(qout);
@@ -167,8 +168,24 @@ void launchStage(dg_context* c, int stage, const void* qin, void* qout, void* uo
     a.dt = static_cast<T>(c->curDt);
     a.stage = stage;
     a.ph = makePhysics<T>(c->phys);
-    hipLaunchKernelGGL((dgk::stageKernel<DIM, P, NF, T, COMP>), dim3(static_cast<unsigned>(c->nblocks)),
-                       dim3(dgk::kBlock), 0, c->stream, a);
+    const dim3 grid(static_cast<unsigned>(c->nblocks)), block(dgk::kBlock);
+    switch (dgk::stageMode(stage)) {
+        case dgk::kModeFirst:
+            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeFirst>), grid, block, 0, c->stream, a);
+            break;
+        case dgk::kModeMid:
+            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeMid>), grid, block, 0, c->stream, a);
+            break;
+        case dgk::kModeFinal:
+            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeFinal>), grid, block, 0, c->stream, a);
+            break;
+        case dgk::kModeEuler:
+            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeEuler>), grid, block, 0, c->stream, a);
+            break;
+        default:
+            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeRHS>), grid, block, 0, c->stream, a);
+            break;
+    }
 }
 
 dgk::SourceArgs sourceArgs(dg_context* c) {
@@ -320,13 +337,23 @@ void launchPost(dg_context* c) {
                        p, sourceArgs(c), static_cast<T*>(c->u), static_cast<T*>(COMP ? c->ulo : nullptr));
 }
 
-template <int DIM, int P, int NF, typename T, bool COMP>
-int stageOccupancy() {
+template <int DIM, int P, int NF, typename T, bool COMP, int MODE>
+int modeOccupancy() {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dgk::stageKernel<DIM, P, NF, T, COMP>, dgk::kBlock, 0) !=
-        hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dgk::stageKernel<DIM, P, NF, T, COMP, MODE>, dgk::kBlock,
+                                                     0) != hipSuccess)
         n = 0;
     return n;
+}
+
+// Resident blocks per CU that every mode of the stage kernel reaches.
+template <int DIM, int P, int NF, typename T, bool COMP>
+int stageOccupancy() {
+    return std::min({modeOccupancy<DIM, P, NF, T, COMP, dgk::kModeFirst>(),
+                     modeOccupancy<DIM, P, NF, T, COMP, dgk::kModeMid>(),
+                     modeOccupancy<DIM, P, NF, T, COMP, dgk::kModeFinal>(),
+                     modeOccupancy<DIM, P, NF, T, COMP, dgk::kModeEuler>(),
+                     modeOccupancy<DIM, P, NF, T, COMP, dgk::kModeRHS>()});
 }
 
 template <int P, int NF, typename T>

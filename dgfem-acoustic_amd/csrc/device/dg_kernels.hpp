@@ -224,38 +224,60 @@ struct StageWaves {
     static constexpr int value = (sizeof(T) == 4 && NF == 4 && NP >= 10 && NP <= 20) ? 7 : 1;
 };
 
-template <int DIM, int P, int NF, typename T, bool COMP>
+// Stage modes of the VALU stage kernel (a template parameter, so that each mode's loads are
+// straight-line code the compiler can issue together).
+enum : int { kModeFirst = 0, kModeMid = 1, kModeFinal = 2, kModeEuler = 3, kModeRHS = 4 };
+
+__host__ __device__ constexpr int stageMode(int stage) {
+    return stage == kStageRK1 ? kModeFirst
+         : (stage == kStageRK2 || stage == kStageRK3) ? kModeMid
+         : stage == kStageRK4 ? kModeFinal
+         : stage == kStageEuler ? kModeEuler : kModeRHS;
+}
+
+// Load scheduling: a block's latency is the number of dependent memory round trips, so every
+// global load of a phase is issued before the first wait. Loads are unconditional, from clamped
+// (always valid) addresses: a load under a branch that merges with a default value makes the
+// compiler wait for it at the merge. Phase 1 (operator tables of the first
+// batch, the element's own stage input, inverse Jacobians, face rows, neighbour codes, and the
+// RK operands consumed at the end) -> LDS -> barrier; phase 2 (the neighbour traces of all of the
+// thread's face nodes) -> fluxes -> barrier; then the contractions and the RK update.
+template <int DIM, int P, int NF, typename T, bool COMP, int MODE>
 __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void stageKernel(StageArgs<T> a) {
     using S = Sizes<DIM, P>;
     using L = OpLayout<DIM, P>;
     constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC, NFT = S::NFT, EB = S::EB;
     constexpr int JJ = DIM * DIM;
-    constexpr bool kOpsInLds = (L::kTotal * sizeof(T)) <= 40 * 1024;
-    // FP32: per-face normals and neighbour codes staged in LDS once per block (measured faster);
-    // FP64: read per face node through the caches (the extra LDS costs more than it saves)
-    constexpr bool kStageFaces = sizeof(T) == 4;
-    constexpr int kFaceLds = kStageFaces ? EB * NFC : 1;
-    constexpr int kOpsLds = kOpsInLds ? L::kTotal : 1;
-    // FP64: persistent blocks loop over batches (the operator table is staged once per block;
-    // measured faster). FP32: one batch per block (higher occupancy; measured faster). The
-    // host sizes the grid accordingly.
+    constexpr int TS = static_cast<int>(sizeof(T));
+    constexpr bool kOpsInLds = (L::kTotal * TS) <= 40 * 1024;
+    // FP64: persistent blocks loop over batches (measured faster); FP32: one batch per block
+    // (higher occupancy; measured faster). The host sizes the grid accordingly.
     constexpr bool kPersistent = sizeof(T) == 8;
+    constexpr bool kNeedU = MODE == kModeMid || MODE == kModeFinal;
+    constexpr bool kFinal = MODE == kModeFinal || MODE == kModeEuler;
+    // per-thread load counts
+    constexpr int kOps16 = kOpsInLds ? (L::kTotal * TS) / 16 : 0;         // 16-byte chunks
+    constexpr int kOpsRem = kOpsInLds ? L::kTotal - kOps16 * (16 / TS) : 0;  // trailing elements
+    constexpr int kOpsIt = (kOps16 + kBlock - 1) / kBlock;
+    constexpr int kIIt = (L::kITotal + kBlock - 1) / kBlock;
+    constexpr int kJIt = (EB * JJ + kBlock - 1) / kBlock;
+    constexpr int kFIt = (EB * NFT + kBlock - 1) / kBlock;
+    static_assert(EB * NP <= kBlock && EB * NFC <= kBlock, "one node and one face row per thread");
+    static_assert(kOpsRem < kBlock, "operator table tail");
+    using V4 = typename Vec4<T>::V;
+    typedef uint32_t U4 __attribute__((ext_vector_type(4)));  // 16 raw bytes (no unions: stays in VGPRs)
 
-    __shared__ __align__(16) T sQ[EB * NP * NF];
+    __shared__ __align__(32) T sQ[EB * NP * NF];
     __shared__ T sF[EB * NFT * NF];
     __shared__ T sJ[EB * JJ];
-    __shared__ __align__(16) T sN[kFaceLds * 4];     // per (element, face): unit normal, Fscale
-    __shared__ __align__(16) int32_t sNb[kFaceLds];  // per (element, face): neighbour / boundary code
-    __shared__ __align__(16) uint8_t sCd8[kFaceLds]; // per (element, face): neighbour face | perm << 2
-    __shared__ T sOps[kOpsLds];
+    __shared__ __align__(32) T sN[EB * NFC * 4];     // per (element, face): unit normal, Fscale
+    __shared__ __align__(16) int32_t sNb[EB * NFC];  // per (element, face): neighbour / boundary code
+    __shared__ __align__(16) uint8_t sCd8[EB * NFC]; // per (element, face): neighbour face | perm << 2
+    __shared__ __align__(16) T sOps[kOpsInLds ? L::kTotal : 1];
     __shared__ int32_t sI[L::kITotal];
     __shared__ double sRed[kBlock / 64][5];
 
     const int tid = threadIdx.x;
-    // constant tables, once per block
-    if constexpr (kOpsInLds)
-        for (int x = tid; x < L::kTotal; x += kBlock) sOps[x] = a.ops[x];
-    for (int x = tid; x < L::kITotal; x += kBlock) sI[x] = a.iops[x];
     const T* ops = kOpsInLds ? sOps : a.ops;
     double res[5] = {0, 0, 0, 0, 0};
 
@@ -269,100 +291,150 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
         bEnd = nBatches * (xcd + 1) / 8;
         bStep = perXcd;
     }
-    const bool needU = a.stage == kStageRK2 || a.stage == kStageRK3 || a.stage == kStageRK4;
-    const bool isFinal = a.stage == kStageRK4 || a.stage == kStageEuler;
-    // one batch of EB elements (inlined into either the persistent loop or a single call)
-    auto batch = [&](int64_t bi) {
+
+    auto batch = [&](int64_t bi, bool withTables) {
     const int64_t e0 = bi * EB;
     const int nE = static_cast<int>((a.K - e0) < EB ? (a.K - e0) : EB);
-
-    // prefetch the RK operands of this thread's node (consumed after the contractions)
     const bool active = tid < nE * NP;
     const int64_t myBase = (e0 * NP + tid) * NF;
+    const int64_t ldBase = (e0 * NP + (active ? tid : 0)) * NF;  // clamped load address
+
+    // ---- phase 1: issue every load ----
+    U4 rOps[kOpsIt > 0 ? kOpsIt : 1];
+    T rOpsTail = T(0);
+    int32_t rI[kIIt];
+    if (withTables) {
+#pragma unroll
+        for (int c = 0; c < kOpsIt; ++c) {
+            const int x = tid + c * kBlock;
+            rOps[c] = reinterpret_cast<const U4*>(a.ops)[x < kOps16 ? x : kOps16 - 1];
+        }
+        if (kOpsRem > 0) rOpsTail = a.ops[kOps16 * (16 / TS) + (tid < kOpsRem ? tid : 0)];
+#pragma unroll
+        for (int c = 0; c < kIIt; ++c) {
+            const int x = tid + c * kBlock;
+            rI[c] = a.iops[x < L::kITotal ? x : 0];
+        }
+    }
+    T rq[NF];
+    loadNode<NF>(a.qin + ldBase, rq);
+    T rJ[kJIt];
+#pragma unroll
+    for (int c = 0; c < kJIt; ++c) {
+        const int x = tid + c * kBlock;
+        rJ[c] = a.geo[e0 * JJ + (x < nE * JJ ? x : 0)];
+    }
+    const bool faceRow = tid < nE * NFC;
+    const V4 rN = reinterpret_cast<const V4*>(a.fgeo + e0 * NFC * 4)[faceRow ? tid : 0];
+    const int eRow = tid < nE ? tid : 0;
+    const int fRow = faceRow ? tid : 0;
+    U4 rNb4;
+    uint32_t rCd4 = 0;  // the element's 4 face codes as one word
+    int32_t rNb1 = 0;
+    uint8_t rCd1 = 0;
+    if constexpr (NFC == 4) {
+        rNb4 = reinterpret_cast<const U4*>(a.nbr + e0 * NFC)[eRow];
+        rCd4 = reinterpret_cast<const uint32_t*>(a.code + e0 * NFC)[eRow];
+    } else {
+        rNb1 = a.nbr[e0 * NFC + fRow];
+        rCd1 = a.code[e0 * NFC + fRow];
+    }
+    // RK operands of this thread's node (consumed after the contractions). The compensation
+    // word enters only the final accumulation: stage inputs are the FP32 u + c k (measured: same
+    // error against FP64 after 2000 steps, 4 B/DOF less per stage).
     T pu[NF], pacc[NF], plo[NF], ps[NF], psl[NF];
 #pragma unroll
     for (int k = 0; k < NF; ++k) pu[k] = pacc[k] = plo[k] = ps[k] = psl[k] = T(0);
-    if (active) {
-        if (needU) {
-            loadNodeNT<NF>(a.u + myBase, pu);
-            loadNodeNT<NF>(a.acc + myBase, pacc);
+    if constexpr (kNeedU) {
+        loadNodeNT<NF>(a.u + ldBase, pu);
+        loadNodeNT<NF>(a.acc + ldBase, pacc);
+    }
+    if constexpr (COMP && kFinal) loadNodeNT<NF>(a.ulo + ldBase, plo);
+
+    // ---- LDS ----
+    if (withTables) {
+#pragma unroll
+        for (int c = 0; c < kOpsIt; ++c) {
+            const int x = tid + c * kBlock;
+            if (x < kOps16) reinterpret_cast<U4*>(sOps)[x] = rOps[c];
         }
-        // the compensation word enters only the final accumulation: stage inputs are the FP32
-        // u + c k (measured: same error against FP64 after 2000 steps, 4 B/DOF less per stage)
-        if constexpr (COMP)
-            if (isFinal) loadNodeNT<NF>(a.ulo + myBase, plo);
-        if (isFinal && a.snap) {
-            loadNodeNT<NF>(a.snap + myBase, ps);
-            if constexpr (COMP) loadNodeNT<NF>(a.snaplo + myBase, psl);
+        if (kOpsRem > 0 && tid < kOpsRem) sOps[kOps16 * (16 / TS) + tid] = rOpsTail;
+#pragma unroll
+        for (int c = 0; c < kIIt; ++c) {
+            const int x = tid + c * kBlock;
+            if (x < L::kITotal) sI[x] = rI[c];
         }
     }
-    {
-        const T* src = a.qin + e0 * NP * NF;
-        if constexpr (NF == 4) {
-            using V4 = typename Vec4<T>::V;
-            for (int x = tid; x < nE * NP; x += kBlock)
-                reinterpret_cast<V4*>(sQ)[x] = reinterpret_cast<const V4*>(src)[x];
-        } else {
-            for (int x = tid; x < nE * NP * NF; x += kBlock) sQ[x] = src[x];
-        }
-        const T* g = a.geo + e0 * JJ;
-        for (int x = tid; x < nE * JJ; x += kBlock) sJ[x] = g[x];
+    if (active) storeNode<NF>(&sQ[tid * NF], rq);
+#pragma unroll
+    for (int c = 0; c < kJIt; ++c) {
+        const int x = tid + c * kBlock;
+        if (x < nE * JJ) sJ[x] = rJ[c];
     }
-    // per (element, face): unit normal, Fscale, neighbour and code, staged once for all the
-    // face nodes of the block
-    if constexpr (kStageFaces) {
-        // rows of 4 (normal + Fscale; 4 neighbours / codes of one element for tets) as vectors
-        using V4 = typename Vec4<T>::V;
-        const V4* fg = reinterpret_cast<const V4*>(a.fgeo + e0 * NFC * 4);
-        for (int x = tid; x < nE * NFC; x += kBlock) reinterpret_cast<V4*>(sN)[x] = fg[x];
-        if constexpr (NFC == 4) {
-            for (int x = tid; x < nE; x += kBlock) {
-                reinterpret_cast<int4*>(sNb)[x] = reinterpret_cast<const int4*>(a.nbr + e0 * NFC)[x];
-                reinterpret_cast<uchar4*>(sCd8)[x] = reinterpret_cast<const uchar4*>(a.code + e0 * NFC)[x];
-            }
-        } else {
-            for (int x = tid; x < nE * NFC; x += kBlock) {
-                sNb[x] = a.nbr[e0 * NFC + x];
-                sCd8[x] = a.code[e0 * NFC + x];
-            }
+    if (faceRow) reinterpret_cast<V4*>(sN)[tid] = rN;
+    if constexpr (NFC == 4) {
+        if (tid < nE) {
+            reinterpret_cast<U4*>(sNb)[tid] = rNb4;
+            reinterpret_cast<uint32_t*>(sCd8)[tid] = rCd4;
+        }
+    } else {
+        if (faceRow) {
+            sNb[tid] = rNb1;
+            sCd8[tid] = rCd1;
         }
     }
     __syncthreads();
 
-    // ---- numerical fluxes at face nodes, scaled by Fscale ----
-    for (int it = tid; it < nE * NFT; it += kBlock) {
-        const int el = it / NFT, fa = it - el * NFT;
-        const int f = fa / NFP, an = fa - f * NFP;
-        const int64_t efg = (e0 + el) * NFC + f;
-        const int efl = kStageFaces ? el * NFC + f : 0;
-        const T* nf = kStageFaces ? &sN[efl * 4] : &a.fgeo[efg * 4];
-        const T nv[3] = {nf[0], nf[1], nf[2]};
-        const T fs = nf[3];
-        const int own = sI[L::kMask + f * NFP + an];
-        T qm[NF], qp[NF], pi[NF];
-        loadNode<NF>(&sQ[(el * NP + own) * NF], qm);
-        const int nb = kStageFaces ? sNb[efl] : a.nbr[efg];
-        if (nb >= 0) {
-            const int cd = kStageFaces ? sCd8[efl] : a.code[efg];
+    // ---- phase 2: neighbour traces of all of this thread's face nodes, then the fluxes ----
+    // (boundary faces and idle threads read the batch's first node: a valid address whose
+    // value the flux does not use)
+    T qp[kFIt][NF];
+#pragma unroll
+    for (int c = 0; c < kFIt; ++c) {
+        const int it = tid + c * kBlock;
+        const T* src = a.qin + e0 * NP * NF;
+        if (it < nE * NFT) {
+            const int el = it / NFT, fa = it - el * NFT;
+            const int f = fa / NFP, an = fa - f * NFP;
+            const int efl = el * NFC + f;
+            const int nb = sNb[efl];
+            const int cd = sCd8[efl];
             const int fp = cd & 3, pc = cd >> 2;
             const int nn = sI[L::kMask + fp * NFP + sI[L::kPerm + pc * NFP + an]];
-            loadNode<NF>(a.qin + (static_cast<int64_t>(nb) * NP + nn) * NF, qp);
-        } else if (nb <= -3) {
-            loadNode<NF>(a.ghost + (static_cast<int64_t>(-3 - nb) * NFP + an) * NF, qp);
-        } else {
-#pragma unroll
-            for (int k = 0; k < NF; ++k) qp[k] = T(0);
+            if (nb >= 0) src = a.qin + (static_cast<int64_t>(nb) * NP + nn) * NF;
+            else if (nb <= -3) src = a.ghost + (static_cast<int64_t>(-3 - nb) * NFP + an) * NF;
         }
-        numericalFlux<DIM, NF>(a.ph, nv, nb <= -3 ? 0 : nb, qm, qp, pi);
+        loadNode<NF>(src, qp[c]);
+    }
+    // snapshot for the residuals, after the gathers (its wait then coincides with theirs)
+    if constexpr (kFinal) {
+        loadNodeNT<NF>((a.snap ? a.snap : a.qin) + ldBase, ps);
+        if constexpr (COMP) loadNodeNT<NF>((a.snaplo ? a.snaplo : a.qin) + ldBase, psl);
+    }
 #pragma unroll
-        for (int k = 0; k < NF; ++k) sF[(el * NFT + fa) * NF + k] = fs * pi[k];
+    for (int c = 0; c < kFIt; ++c) {
+        const int it = tid + c * kBlock;
+        if (it < nE * NFT) {
+            const int el = it / NFT, fa = it - el * NFT;
+            const int f = fa / NFP, an = fa - f * NFP;
+            const int efl = el * NFC + f;
+            const T* nf = &sN[efl * 4];
+            const T nv[3] = {nf[0], nf[1], nf[2]};
+            const T fs = nf[3];
+            const int nb = sNb[efl];
+            const int own = sI[L::kMask + f * NFP + an];
+            T qm[NF], pi[NF];
+            loadNode<NF>(&sQ[(el * NP + own) * NF], qm);
+            numericalFlux<DIM, NF>(a.ph, nv, nb <= -3 ? 0 : nb, qm, qp[c], pi);
+#pragma unroll
+            for (int k = 0; k < NF; ++k) sF[(el * NFT + fa) * NF + k] = fs * pi[k];
+        }
     }
     __syncthreads();
 
     // ---- volume contraction, flux-Jacobian fold, lift, register update ----
-    if (tid < nE * NP) {
+    if (active) {
         const int el = tid / NP, i = tid - el * NP;
-        const int64_t node = (e0 + el) * NP + i;
         T Dq[DIM][NF];
 #pragma unroll
         for (int r = 0; r < DIM; ++r)
@@ -422,21 +494,21 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
             for (int k = 0; k < NF; ++k) rhs[k] -= l * f0[fa * NF + k];
         }
 
-        const int64_t base = node * NF;
+        const int64_t base = myBase;
         const T dt = a.dt;
-        if (a.stage == kStageRHS) {
+        if constexpr (MODE == kModeRHS) {
             T o[NF];
 #pragma unroll
             for (int k = 0; k < NF; ++k) o[k] = dt * rhs[k];
             storeNode<NF>(a.qout + base, o);
-        } else if (a.stage <= kStageRK3) {
-            const T half = (a.stage == kStageRK3) ? T(1) : T(0.5);
+        } else if constexpr (MODE == kModeFirst || MODE == kModeMid) {
+            const T half = (MODE == kModeMid && a.stage == kStageRK3) ? T(1) : T(0.5);
             T accn[NF], qn[NF];
 #pragma unroll
             for (int k = 0; k < NF; ++k) {
                 const T kk = dt * rhs[k];
                 T uv;
-                if (a.stage == kStageRK1) {
+                if constexpr (MODE == kModeFirst) {
                     uv = q0[i * NF + k];
                     accn[k] = kk;
                 } else {
@@ -454,8 +526,8 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
 #pragma unroll
             for (int k = 0; k < NF; ++k) {
                 const T kk = dt * rhs[k];
-                const T delta = (a.stage == kStageRK4) ? (pacc[k] + kk) / T(6) : kk;
-                const T uv = (a.stage == kStageRK4) ? pu[k] : q0[i * NF + k];
+                const T delta = (MODE == kModeFinal) ? (pacc[k] + kk) / T(6) : kk;
+                const T uv = (MODE == kModeFinal) ? pu[k] : q0[i * NF + k];
                 if constexpr (COMP) {
                     T s, e, t, e2;
                     twoSum(plo[k], delta, t, e2);
@@ -495,26 +567,30 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
     }
     };
     if constexpr (kPersistent) {
+        bool first = true;
         for (int64_t bi = bFirst; bi < bEnd; bi += bStep) {
-            batch(bi);
+            batch(bi, first);
+            first = false;
             __syncthreads();  // the next batch overwrites the staged state and fluxes
         }
     } else {
-        if (bFirst < bEnd) batch(bFirst);
+        if (bFirst < bEnd) batch(bFirst, true);
     }
-    if ((a.stage == kStageRK4 || a.stage == kStageEuler) && a.snap) {
-        // block reduction of the residual partial sums (deterministic order)
+    if constexpr (kFinal) {
+        if (a.snap) {
+            // block reduction of the residual partial sums (deterministic order)
 #pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            double v = res[k];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-            if ((tid & 63) == 0) sRed[tid >> 6][k] = v;
-        }
-        __syncthreads();
-        if (tid < 5) {
-            double s = 0.0;
-            for (int w = 0; w < kBlock / 64; ++w) s += sRed[w][tid];
-            a.resPartial[static_cast<int64_t>(blockIdx.x) * 5 + tid] = s;
+            for (int k = 0; k < 5; ++k) {
+                double v = res[k];
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+                if ((tid & 63) == 0) sRed[tid >> 6][k] = v;
+            }
+            __syncthreads();
+            if (tid < 5) {
+                double s = 0.0;
+                for (int w = 0; w < kBlock / 64; ++w) s += sRed[w][tid];
+                a.resPartial[static_cast<int64_t>(blockIdx.x) * 5 + tid] = s;
+            }
         }
     }
 }
