@@ -233,6 +233,7 @@ void launchStageWave(dg_context* c, int stage, const void* qin, void* qout, void
 
 template <int DIM, int P, typename T, bool COMP>
 void launchStageMM(dg_context* c, int stage, const void* qin, void* qout, void* uout, bool snap) {
+    using dgk::stageKernelMM;
     dgk::StageArgs<T> a{};
     a.qin = static_cast<const T*>(qin);
     a.qout = static_cast<T*>(qout);
@@ -254,8 +255,25 @@ void launchStageMM(dg_context* c, int stage, const void* qin, void* qout, void* 
     a.dt = static_cast<T>(c->curDt);
     a.stage = stage;
     a.ph = makePhysics<T>(c->phys);
-    hipLaunchKernelGGL((dgk::stageKernelMM<DIM, P, T, COMP>), dim3(static_cast<unsigned>(c->nblocks)),
-                       dim3(dgk::kBlock), 0, c->stream, a, static_cast<const T*>(c->afrag));
+    const T* af = static_cast<const T*>(c->afrag);
+    const dim3 grid(static_cast<unsigned>(c->nblocks)), block(dgk::kBlock);
+    switch (dgk::stageMode(stage)) {
+        case dgk::kModeFirst:
+            hipLaunchKernelGGL((stageKernelMM<DIM, P, T, COMP, dgk::kModeFirst>), grid, block, 0, c->stream, a, af);
+            break;
+        case dgk::kModeMid:
+            hipLaunchKernelGGL((stageKernelMM<DIM, P, T, COMP, dgk::kModeMid>), grid, block, 0, c->stream, a, af);
+            break;
+        case dgk::kModeFinal:
+            hipLaunchKernelGGL((stageKernelMM<DIM, P, T, COMP, dgk::kModeFinal>), grid, block, 0, c->stream, a, af);
+            break;
+        case dgk::kModeEuler:
+            hipLaunchKernelGGL((stageKernelMM<DIM, P, T, COMP, dgk::kModeEuler>), grid, block, 0, c->stream, a, af);
+            break;
+        default:
+            hipLaunchKernelGGL((stageKernelMM<DIM, P, T, COMP, dgk::kModeRHS>), grid, block, 0, c->stream, a, af);
+            break;
+    }
 }
 
 template <int DIM, int P>

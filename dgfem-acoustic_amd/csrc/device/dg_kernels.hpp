@@ -108,12 +108,14 @@ __device__ inline void numericalFlux(const Physics<T>& ph, const T n[3], int nb,
                                      T* pi) {
     const T vn0 = ph.v0[0] * n[0] + ph.v0[1] * n[1] + ph.v0[2] * n[2];
     if (nb >= 0) {
-        T fm[NF], fp[NF];
-        normalFlux<NF>(ph, n, vn0, qm, fm);
-        normalFlux<NF>(ph, n, vn0, qp, fp);
+        // F is linear in q: n.(F(q-) + F(q+))/2 = n.F(q- + q+)/2
+        T qs[NF], fs[NF];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) qs[k] = qm[k] + qp[k];
+        normalFlux<NF>(ph, n, vn0, qs, fs);
         const T hc = T(0.5) * ph.c0;
 #pragma unroll
-        for (int k = 0; k < NF; ++k) pi[k] = T(0.5) * (fm[k] + fp[k]) + hc * (qm[k] - qp[k]);
+        for (int k = 0; k < NF; ++k) pi[k] = T(0.5) * fs[k] + hc * (qm[k] - qp[k]);
     } else if (nb == -2) {
         T q[NF];
         const T vn = n[0] * qm[1] + n[1] * qm[2] + n[2] * qm[3];
@@ -224,6 +226,9 @@ struct StageWaves {
     static constexpr int value = (sizeof(T) == 4 && NF == 4 && NP >= 10 && NP <= 20) ? 7 : 1;
 };
 
+// Padded LDS row stride (in elements) for per-element blocks of n values: == 4 (mod 16).
+__host__ __device__ constexpr int ldsStride(int n) { return n + ((4 - n % 16) + 16) % 16; }
+
 // Stage modes of the VALU stage kernel (a template parameter, so that each mode's loads are
 // straight-line code the compiler can issue together).
 enum : int { kModeFirst = 0, kModeMid = 1, kModeFinal = 2, kModeEuler = 3, kModeRHS = 4 };
@@ -267,8 +272,11 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
     using V4 = typename Vec4<T>::V;
     typedef uint32_t U4 __attribute__((ext_vector_type(4)));  // 16 raw bytes (no unions: stays in VGPRs)
 
-    __shared__ __align__(32) T sQ[EB * NP * NF];
-    __shared__ T sF[EB * NFT * NF];
+    // per-element strides of the staged state and fluxes padded to 4 (mod 16) elements: the
+    // 3-4 elements a wave spans then start in different LDS banks
+    constexpr int QS = ldsStride(NP * NF), FS = ldsStride(NFT * NF);
+    __shared__ __align__(32) T sQ[EB * QS];
+    __shared__ __align__(32) T sF[EB * FS];
     __shared__ T sJ[EB * JJ];
     __shared__ __align__(32) T sN[EB * NFC * 4];     // per (element, face): unit normal, Fscale
     __shared__ __align__(16) int32_t sNb[EB * NFC];  // per (element, face): neighbour / boundary code
@@ -365,7 +373,10 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
             if (x < L::kITotal) sI[x] = rI[c];
         }
     }
-    if (active) storeNode<NF>(&sQ[tid * NF], rq);
+    if (active) {
+        const int el = tid / NP;
+        storeNode<NF>(&sQ[el * QS + (tid - el * NP) * NF], rq);
+    }
 #pragma unroll
     for (int c = 0; c < kJIt; ++c) {
         const int x = tid + c * kBlock;
@@ -424,10 +435,10 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
             const int nb = sNb[efl];
             const int own = sI[L::kMask + f * NFP + an];
             T qm[NF], pi[NF];
-            loadNode<NF>(&sQ[(el * NP + own) * NF], qm);
+            loadNode<NF>(&sQ[el * QS + own * NF], qm);
             numericalFlux<DIM, NF>(a.ph, nv, nb <= -3 ? 0 : nb, qm, qp[c], pi);
 #pragma unroll
-            for (int k = 0; k < NF; ++k) sF[(el * NFT + fa) * NF + k] = fs * pi[k];
+            for (int k = 0; k < NF; ++k) sF[el * FS + fa * NF + k] = fs * pi[k];
         }
     }
     __syncthreads();
@@ -440,7 +451,7 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
         for (int r = 0; r < DIM; ++r)
 #pragma unroll
             for (int k = 0; k < NF; ++k) Dq[r][k] = T(0);
-        const T* q0 = &sQ[el * NP * NF];
+        const T* q0 = &sQ[el * QS];
 #pragma unroll 4
         for (int j = 0; j < NP; ++j) {
             T q[NF];
@@ -486,7 +497,7 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
             }
             if constexpr (NF == 5) rhs[4] = rr;
         }
-        const T* f0 = &sF[el * NFT * NF];
+        const T* f0 = &sF[el * FS];
 #pragma unroll 4
         for (int fa = 0; fa < NFT; ++fa) {
             const T l = ops[L::kL + fa * NP + i];
@@ -949,7 +960,7 @@ struct BlockMmShape {
     static constexpr bool kSupported = M::kSupported && Sizes<DIM, P>::NP <= KROWS;
 };
 
-template <int DIM, int P, typename T, bool COMP>
+template <int DIM, int P, typename T, bool COMP, int MODE>
 __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T* __restrict__ afrag) {
     using S = Sizes<DIM, P>;
     using L = OpLayout<DIM, P>;
@@ -957,14 +968,27 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
     using B = BlockMmShape<DIM, P>;
     using MT_ = MfmaTraits<T>;
     using V4 = typename Vec4<T>::V;
+    typedef uint32_t U4 __attribute__((ext_vector_type(4)));
     constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC, NFT = S::NFT, NF = 4, EB = B::EB;
     constexpr int JJ = DIM * DIM;
     constexpr int LIFT0 = 4 * M::KSV;  // first B row of the lift part
+    constexpr bool kNeedU = MODE == kModeMid || MODE == kModeFinal;
+    constexpr bool kFinal = MODE == kModeFinal || MODE == kModeEuler;
+    // per-thread item counts (loads of a phase are all issued before the first wait; addresses of
+    // idle items are clamped to valid ones so that no load sits under a branch)
+    constexpr int kQIt = (EB * NP + kBlock - 1) / kBlock;    // staged nodes
+    constexpr int kJIt = (EB * JJ + kBlock - 1) / kBlock;    // inverse-Jacobian entries
+    constexpr int kRIt = (EB * NFC + kBlock - 1) / kBlock;   // face rows
+    constexpr int kIIt = (L::kITotal + kBlock - 1) / kBlock; // index-table entries
+    constexpr int kFIt = (EB * NFT + kBlock - 1) / kBlock;   // face nodes
+    constexpr int NIT = (4 * NP + 63) / 64;                  // update items per lane (it = lane + 64 c)
+    constexpr int padV = LIFT0 - M::KV, padL = B::KROWS - LIFT0 - NFT;
+    constexpr int kPadIt = ((padV + padL) * 64 + kBlock - 1) / kBlock;
 
-    __shared__ __align__(16) T sB[B::kBFloats];
-    __shared__ __align__(16) T sQ[EB * NP * NF];
+    __shared__ __align__(32) T sB[B::kBFloats];
+    __shared__ __align__(32) T sQ[EB * NP * NF];
     __shared__ T sJ[EB * JJ];
-    __shared__ __align__(16) T sN[EB * NFC * 4];
+    __shared__ __align__(32) T sN[EB * NFC * 4];
     __shared__ int32_t sNb[EB * NFC];
     __shared__ int32_t sCd[EB * NFC];
     __shared__ int32_t sI[L::kITotal];
@@ -974,12 +998,125 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
     const int64_t blk = xcdRemap(blockIdx.x, gridDim.x);
     const int64_t e0 = blk * EB;
     const int nE = static_cast<int>((a.K - e0) < EB ? (a.K - e0) : EB);
-    const int stage = a.stage;
-    const bool needU = stage == kStageRK2 || stage == kStageRK3 || stage == kStageRK4;
-    const bool isFinal = stage == kStageRK4 || stage == kStageEuler;
+    const int nEw = nE - 4 * wave;  // elements of this wave's column tile present in the block
 
-    // A fragments of this lane (constant operators), resident for the whole block; stored
-    // lane-contiguous ([lane][mt][ks]) so that they load as 16-byte vectors
+    // ---- phase 1: issue every load ----
+    T rq[kQIt][NF];
+#pragma unroll
+    for (int c = 0; c < kQIt; ++c) {
+        const int x = tid + c * kBlock;
+        loadNode<NF>(a.qin + (e0 * NP + (x < nE * NP ? x : 0)) * NF, rq[c]);
+    }
+    T rJ[kJIt];
+#pragma unroll
+    for (int c = 0; c < kJIt; ++c) {
+        const int x = tid + c * kBlock;
+        rJ[c] = a.geo[e0 * JJ + (x < nE * JJ ? x : 0)];
+    }
+    V4 rN[kRIt];
+    int32_t rNb[kRIt], rCd[kRIt];
+#pragma unroll
+    for (int c = 0; c < kRIt; ++c) {
+        const int x = tid + c * kBlock;
+        const int xr = x < nE * NFC ? x : 0;
+        rN[c] = reinterpret_cast<const V4*>(a.fgeo + e0 * NFC * 4)[xr];
+        rNb[c] = a.nbr[e0 * NFC + xr];
+        rCd[c] = a.code[e0 * NFC + xr];
+    }
+    int32_t rI[kIIt];
+#pragma unroll
+    for (int c = 0; c < kIIt; ++c) {
+        const int x = tid + c * kBlock;
+        rI[c] = a.iops[x < L::kITotal ? x : 0];
+    }
+    // RK operands of this lane's update items (node-major over the wave's 4 elements)
+    int64_t uNode[NIT];
+    bool uOk[NIT];
+    T pu[NIT][NF], pacc[NIT][NF], plo[NIT][NF], ps[NIT][NF], psl[NIT][NF];
+#pragma unroll
+    for (int c = 0; c < NIT; ++c) {
+        const int it = lane + 64 * c;
+        const int el = it / NP, i = it - el * NP;
+        uOk[c] = it < 4 * NP && el < nEw;
+        uNode[c] = uOk[c] ? (e0 + 4 * wave + el) * NP + i : e0 * NP;
+#pragma unroll
+        for (int k = 0; k < NF; ++k) pu[c][k] = pacc[c][k] = plo[c][k] = ps[c][k] = psl[c][k] = T(0);
+        if constexpr (kNeedU) {
+            loadNodeNT<NF>(a.u + uNode[c] * NF, pu[c]);
+            loadNodeNT<NF>(a.acc + uNode[c] * NF, pacc[c]);
+        }
+        if constexpr (COMP && kFinal) loadNodeNT<NF>(a.ulo + uNode[c] * NF, plo[c]);
+    }
+
+    // ---- LDS; zero the B padding rows ----
+#pragma unroll
+    for (int c = 0; c < kQIt; ++c) {
+        const int x = tid + c * kBlock;
+        if (x < nE * NP) storeNode<NF>(&sQ[x * NF], rq[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < kJIt; ++c) {
+        const int x = tid + c * kBlock;
+        if (x < nE * JJ) sJ[x] = rJ[c];
+    }
+#pragma unroll
+    for (int c = 0; c < kRIt; ++c) {
+        const int x = tid + c * kBlock;
+        if (x < nE * NFC) {
+            reinterpret_cast<V4*>(sN)[x] = rN[c];
+            sNb[x] = rNb[c];
+            sCd[x] = rCd[c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < kIIt; ++c) {
+        const int x = tid + c * kBlock;
+        if (x < L::kITotal) sI[x] = rI[c];
+    }
+#pragma unroll
+    for (int c = 0; c < kPadIt; ++c) {
+        const int x = tid + c * kBlock;
+        if (x < (padV + padL) * 64) {
+            const int pr = x / 64, col = x - pr * 64;
+            const int kk = pr < padV ? M::KV + pr : LIFT0 + NFT + (pr - padV);
+            sB[((kk >> 2) * 4 + (col >> 4)) * 64 + (kk & 3) * 16 + (col & 15)] = T(0);
+        }
+    }
+    __syncthreads();
+
+    // B fragment index of (row kk, column 4*e + field 0); the 4 fields are contiguous
+    auto bIndex = [](int kk, int e) { return ((kk >> 2) * 4 + (e >> 2)) * 64 + (kk & 3) * 16 + (e & 3) * 4; };
+
+    // ---- phase 2: neighbour traces of all of this thread's face nodes ----
+    // (boundary faces and idle items read the block's first node: a valid address whose value
+    // the flux does not use)
+    T qp[kFIt][NF];
+#pragma unroll
+    for (int c = 0; c < kFIt; ++c) {
+        const int it = tid + c * kBlock;
+        const T* src = a.qin + e0 * NP * NF;
+        if (it < nE * NFT) {
+            const int e = it / NFT, fa = it - e * NFT;
+            const int f = fa / NFP, an = fa - f * NFP;
+            const int ef = e * NFC + f;
+            const int nb = sNb[ef];
+            const int cd = sCd[ef];
+            const int fp = cd & 3, pc = cd >> 2;
+            const int nn = sI[L::kMask + fp * NFP + sI[L::kPerm + pc * NFP + an]];
+            if (nb >= 0) src = a.qin + (static_cast<int64_t>(nb) * NP + nn) * NF;
+            else if (nb <= -3) src = a.ghost + (static_cast<int64_t>(-3 - nb) * NFP + an) * NF;
+        }
+        loadNode<NF>(src, qp[c]);
+    }
+    if constexpr (kFinal) {
+#pragma unroll
+        for (int c = 0; c < NIT; ++c) {
+            loadNodeNT<NF>((a.snap ? a.snap : a.qin) + uNode[c] * NF, ps[c]);
+            if constexpr (COMP) loadNodeNT<NF>((a.snaplo ? a.snaplo : a.qin) + uNode[c] * NF, psl[c]);
+        }
+    }
+    // A fragments of this lane (constant operators), stored lane-contiguous ([lane][mt][ks]) so
+    // that they load as 16-byte vectors; issued here so that they arrive with the traces
     T af[B::kAPerLane];
     {
         const V4* src = reinterpret_cast<const V4*>(afrag + lane * B::kAPerLane);
@@ -993,91 +1130,60 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
         }
     }
 
-    // 1. stage state, geometry, face data, index tables; zero the B padding rows
-    {
-        const V4* q4 = reinterpret_cast<const V4*>(a.qin + e0 * NP * NF);
-        for (int x = tid; x < nE * NP; x += kBlock) reinterpret_cast<V4*>(sQ)[x] = q4[x];
-        for (int x = tid; x < nE * JJ; x += kBlock) sJ[x] = a.geo[e0 * JJ + x];
-        for (int x = tid; x < nE * NFC * 4; x += kBlock) sN[x] = a.fgeo[e0 * NFC * 4 + x];
-        for (int x = tid; x < nE * NFC; x += kBlock) {
-            sNb[x] = a.nbr[e0 * NFC + x];
-            sCd[x] = a.code[e0 * NFC + x];
-        }
-        for (int x = tid; x < L::kITotal; x += kBlock) sI[x] = a.iops[x];
-        constexpr int padV = LIFT0 - M::KV, padL = B::KROWS - LIFT0 - NFT;
-        for (int x = tid; x < (padV + padL) * 64; x += kBlock) {
-            const int pr = x / 64, col = x - pr * 64;
-            const int kk = pr < padV ? M::KV + pr : LIFT0 + NFT + (pr - padV);
-            sB[((kk >> 2) * 4 + (col >> 4)) * 64 + (kk & 3) * 16 + (col & 15)] = T(0);
-        }
-    }
-    __syncthreads();
-
-    // B fragment index of (row kk, column 4*e + field 0); the 4 fields are contiguous
-    auto bIndex = [](int kk, int e) { return ((kk >> 2) * 4 + (e >> 2)) * 64 + (kk & 3) * 16 + (e & 3) * 4; };
-
-    // 2a. volume operand G_r[j] (one item per element node)
-    for (int it = tid; it < nE * NP; it += kBlock) {
-        const int e = it / NP, j = it - e * NP;
-        const V4 q = reinterpret_cast<const V4*>(sQ)[it];
-        const T* J = &sJ[e * JJ];
+    // 2a. volume operand G_r[j] = sum_x J_rx F_x(q_j), F_x = (v0_x p + rho0 c0^2 v_x, v0_x v + e_x p / rho0)
+    //     (one item per element node; overlaps the traces in flight)
 #pragma unroll
-        for (int r = 0; r < DIM; ++r) {
-            // sum_x J_rx F_x(q): F_x = (v0_x p + rho0 c0^2 v_x, v0_x v + e_x p / rho0)
-            T w = T(0), jv = T(0);
-            T jx[3] = {T(0), T(0), T(0)};
+    for (int c = 0; c < kQIt; ++c) {
+        const int it = tid + c * kBlock;
+        if (it < nE * NP) {
+            const int e = it / NP, j = it - e * NP;
+            const T* q = rq[c];
+            const T* J = &sJ[e * JJ];
 #pragma unroll
-            for (int x = 0; x < DIM; ++x) {
-                jx[x] = J[r * DIM + x];
-                w += jx[x] * a.ph.v0[x];
+            for (int r = 0; r < DIM; ++r) {
+                T w = T(0);
+                T jx[3] = {T(0), T(0), T(0)};
+#pragma unroll
+                for (int x = 0; x < DIM; ++x) {
+                    jx[x] = J[r * DIM + x];
+                    w += jx[x] * a.ph.v0[x];
+                }
+                T jv = jx[0] * q[1];
+                if (DIM > 1) jv += jx[1] * q[2];
+                if (DIM > 2) jv += jx[DIM > 2 ? 2 : 0] * q[3];
+                const T pq = q[0] * a.ph.irho0;
+                T g[NF];
+                g[0] = w * q[0] + a.ph.rc2 * jv;
+                g[1] = w * q[1] + jx[0] * pq;
+                g[2] = w * q[2] + jx[1] * pq;
+                g[3] = w * q[3] + jx[2] * pq;
+                storeNode<NF>(&sB[bIndex(r * NP + j, e)], g);
             }
-            jv = jx[0] * q.y;
-            if (DIM > 1) jv += jx[1] * q.z;
-            if (DIM > 2) jv += jx[DIM > 2 ? 2 : 0] * q.w;
-            V4 g;
-            g.x = w * q.x + a.ph.rc2 * jv;
-            g.y = w * q.y + jx[0] * q.x * a.ph.irho0;
-            g.z = w * q.z + jx[1] * q.x * a.ph.irho0;
-            g.w = w * q.w + jx[2] * q.x * a.ph.irho0;
-            reinterpret_cast<V4*>(sB)[bIndex(r * NP + j, e) >> 2] = g;
         }
     }
     // 2b. scaled numerical fluxes (one item per element face node)
-    for (int it = tid; it < nE * NFT; it += kBlock) {
-        const int e = it / NFT, fa = it - e * NFT;
-        const int f = fa / NFP, an = fa - f * NFP;
-        const int ef = e * NFC + f;
-        const T nv[3] = {sN[ef * 4 + 0], sN[ef * 4 + 1], sN[ef * 4 + 2]};
-        const T fs = sN[ef * 4 + 3];
-        const int own = sI[L::kMask + f * NFP + an];
-        T qm[NF], qp[NF], pi[NF];
-        {
-            const V4 v = reinterpret_cast<const V4*>(sQ)[e * NP + own];
-            qm[0] = v.x; qm[1] = v.y; qm[2] = v.z; qm[3] = v.w;
+#pragma unroll
+    for (int c = 0; c < kFIt; ++c) {
+        const int it = tid + c * kBlock;
+        if (it < nE * NFT) {
+            const int e = it / NFT, fa = it - e * NFT;
+            const int f = fa / NFP, an = fa - f * NFP;
+            const int ef = e * NFC + f;
+            const T nv[3] = {sN[ef * 4 + 0], sN[ef * 4 + 1], sN[ef * 4 + 2]};
+            const T fs = sN[ef * 4 + 3];
+            const int own = sI[L::kMask + f * NFP + an];
+            const int nb = sNb[ef];
+            T qm[NF], pi[NF], out[NF];
+            loadNode<NF>(&sQ[(e * NP + own) * NF], qm);
+            numericalFlux<DIM, NF>(a.ph, nv, nb <= -3 ? 0 : nb, qm, qp[c], pi);
+#pragma unroll
+            for (int k = 0; k < NF; ++k) out[k] = fs * pi[k];
+            storeNode<NF>(&sB[bIndex(LIFT0 + fa, e)], out);
         }
-        const int nb = sNb[ef];
-        if (nb >= 0) {
-            const int cd = sCd[ef];
-            const int fp = cd & 3, pc = cd >> 2;
-            const int nn = sI[L::kMask + fp * NFP + sI[L::kPerm + pc * NFP + an]];
-            const V4 v = reinterpret_cast<const V4*>(a.qin)[static_cast<int64_t>(nb) * NP + nn];
-            qp[0] = v.x; qp[1] = v.y; qp[2] = v.z; qp[3] = v.w;
-        } else if (nb <= -3) {
-            const V4 v = reinterpret_cast<const V4*>(a.ghost)[static_cast<int64_t>(-3 - nb) * NFP + an];
-            qp[0] = v.x; qp[1] = v.y; qp[2] = v.z; qp[3] = v.w;
-        } else {
-            qp[0] = qp[1] = qp[2] = qp[3] = T(0);
-        }
-        numericalFlux<DIM, NF>(a.ph, nv, nb <= -3 ? 0 : nb, qm, qp, pi);
-        V4 out;
-        out.x = fs * pi[0]; out.y = fs * pi[1]; out.z = fs * pi[2]; out.w = fs * pi[3];
-        reinterpret_cast<V4*>(sB)[bIndex(LIFT0 + fa, e) >> 2] = out;
     }
-    constexpr int NIT = (4 * NP + 63) / 64;  // update items per lane (it = lane + 64 c)
-    const int nEw = nE - 4 * wave;           // elements of this wave present in the block
     __syncthreads();
 
-    // 3. matrix core: this wave's column tile (elements 4*wave .. 4*wave+3)
+    // ---- 3. matrix core: this wave's column tile (elements 4*wave .. 4*wave+3) ----
     typename MT_::V acc[M::MT][2];
 #pragma unroll
     for (int mt = 0; mt < M::MT; ++mt) acc[mt][0] = acc[mt][1] = typename MT_::V{T(0), T(0), T(0), T(0)};
@@ -1094,7 +1200,7 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
     for (int mt = 0; mt < M::MT; ++mt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int row = 16 * mt + MT_::row(lane, r) - 0;
+            const int row = 16 * mt + MT_::row(lane, r);
             if (row < NP) {
                 const T v = acc[mt][0][r] + acc[mt][1][r];
                 Bw[((row >> 2) * 4) * 64 + (row & 3) * 16 - lane + (lane & 15)] = v;
@@ -1102,68 +1208,53 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
         }
     waveSync();
 
-    // 4b. RK update, node-major over this wave's 4 elements
+    // ---- 4b. RK update, node-major over this wave's 4 elements ----
     double res[5] = {0, 0, 0, 0, 0};
 #pragma unroll
     for (int c = 0; c < NIT; ++c) {
+        if (!uOk[c]) continue;
         const int it = lane + 64 * c;
         const int el = it / NP, i = it - el * NP;
-        if (it >= 4 * NP || el >= nEw) continue;
         const int eb = 4 * wave + el;
-        const V4 rv = reinterpret_cast<const V4*>(sB)[(((i >> 2) * 4 + wave) * 64 + (i & 3) * 16 + el * 4) >> 2];
-        const T rhs[4] = {rv.x, rv.y, rv.z, rv.w};
-        const int64_t node = (e0 + eb) * NP + i;
-        const V4 q0v = reinterpret_cast<const V4*>(sQ)[eb * NP + i];
-        const T q0[4] = {q0v.x, q0v.y, q0v.z, q0v.w};
-        T pu[4] = {T(0), T(0), T(0), T(0)}, pacc[4] = {T(0), T(0), T(0), T(0)}, plo[4] = {T(0), T(0), T(0), T(0)};
-        if (needU) {
-            const V4 x = reinterpret_cast<const V4*>(a.u)[node];
-            const V4 y = reinterpret_cast<const V4*>(a.acc)[node];
-            pu[0] = x.x; pu[1] = x.y; pu[2] = x.z; pu[3] = x.w;
-            pacc[0] = y.x; pacc[1] = y.y; pacc[2] = y.z; pacc[3] = y.w;
-        }
-        if constexpr (COMP) {
-            if (isFinal) {
-                const V4 x = reinterpret_cast<const V4*>(a.ulo)[node];
-                plo[0] = x.x; plo[1] = x.y; plo[2] = x.z; plo[3] = x.w;
-            }
-        }
+        T rhs[NF], q0[NF];
+        loadNode<NF>(&sB[((i >> 2) * 4 + wave) * 64 + (i & 3) * 16 + el * 4], rhs);
+        loadNode<NF>(&sQ[(eb * NP + i) * NF], q0);
+        const int64_t base = uNode[c] * NF;
         const T dt = a.dt;
-        if (stage == kStageRHS) {
-            V4 o;
-            o.x = dt * rhs[0]; o.y = dt * rhs[1]; o.z = dt * rhs[2]; o.w = dt * rhs[3];
-            reinterpret_cast<V4*>(a.qout)[node] = o;
-        } else if (stage <= kStageRK3) {
-            const T half = (stage == kStageRK3) ? T(1) : T(0.5);
-            T accn[4], qn[4];
+        if constexpr (MODE == kModeRHS) {
+            T o[NF];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < NF; ++k) o[k] = dt * rhs[k];
+            storeNode<NF>(a.qout + base, o);
+        } else if constexpr (MODE == kModeFirst || MODE == kModeMid) {
+            const T half = (MODE == kModeMid && a.stage == kStageRK3) ? T(1) : T(0.5);
+            T accn[NF], qn[NF];
+#pragma unroll
+            for (int k = 0; k < NF; ++k) {
                 const T kk = dt * rhs[k];
                 T uv;
-                if (stage == kStageRK1) {
+                if constexpr (MODE == kModeFirst) {
                     uv = q0[k];
                     accn[k] = kk;
                 } else {
-                    uv = pu[k];
-                    accn[k] = pacc[k] + T(2) * kk;
+                    uv = pu[c][k];
+                    accn[k] = pacc[c][k] + T(2) * kk;
                 }
                 qn[k] = uv + half * kk;
             }
-            V4 oa, oq;
-            oa.x = accn[0]; oa.y = accn[1]; oa.z = accn[2]; oa.w = accn[3];
-            oq.x = qn[0]; oq.y = qn[1]; oq.z = qn[2]; oq.w = qn[3];
-            reinterpret_cast<V4*>(a.acc)[node] = oa;
-            reinterpret_cast<V4*>(a.qout)[node] = oq;
+            storeNodeNT<NF>(a.acc + base, accn);
+            storeNode<NF>(a.qout + base, qn);
         } else {
-            T unew[4], lw[4];
+            // final update: u += (k1 + 2k2 + 2k3 + k4)/6  (RK4)  or  u += k  (Euler)
+            T unew[NF], lw[NF];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < NF; ++k) {
                 const T kk = dt * rhs[k];
-                const T delta = (stage == kStageRK4) ? (pacc[k] + kk) / T(6) : kk;
-                const T uv = (stage == kStageRK4) ? pu[k] : q0[k];
+                const T delta = (MODE == kModeFinal) ? (pacc[c][k] + kk) / T(6) : kk;
+                const T uv = (MODE == kModeFinal) ? pu[c][k] : q0[k];
                 if constexpr (COMP) {
                     T s_, e_, t_, e2;
-                    twoSum(plo[k], delta, t_, e2);
+                    twoSum(plo[c][k], delta, t_, e2);
                     twoSum(uv, t_, s_, e_);
                     const T nlo = e_ + e2;
                     T h, l2;
@@ -1175,29 +1266,16 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
                     lw[k] = T(0);
                 }
             }
-            V4 oh;
-            oh.x = unew[0]; oh.y = unew[1]; oh.z = unew[2]; oh.w = unew[3];
-            reinterpret_cast<V4*>(a.uout)[node] = oh;
-            if constexpr (COMP) {
-                V4 ol;
-                ol.x = lw[0]; ol.y = lw[1]; ol.z = lw[2]; ol.w = lw[3];
-                reinterpret_cast<V4*>(a.ulo)[node] = ol;
-            }
+            storeNodeNT<NF>(a.uout + base, unew);
+            if constexpr (COMP) storeNodeNT<NF>(a.ulo + base, lw);
             if (a.snap) {
-                const V4 sv4 = reinterpret_cast<const V4*>(a.snap)[node];
-                double sv[4] = {static_cast<double>(sv4.x), static_cast<double>(sv4.y), static_cast<double>(sv4.z),
-                                static_cast<double>(sv4.w)};
-                if constexpr (COMP) {
-                    const V4 sl = reinterpret_cast<const V4*>(a.snaplo)[node];
-                    sv[0] += static_cast<double>(sl.x);
-                    sv[1] += static_cast<double>(sl.y);
-                    sv[2] += static_cast<double>(sl.z);
-                    sv[3] += static_cast<double>(sl.w);
-                }
-                double d[4];
+                double sv[4], d[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k)
+                for (int k = 0; k < 4; ++k) {
+                    sv[k] = static_cast<double>(ps[c][k]);
+                    if constexpr (COMP) sv[k] += static_cast<double>(psl[c][k]);
                     d[k] = (static_cast<double>(unew[k]) + static_cast<double>(lw[k])) - sv[k];
+                }
                 const double ic2 = static_cast<double>(a.ph.ic2);
                 const double u0 = static_cast<double>(unew[0]) + static_cast<double>(lw[0]);
                 const double sp = sv[0] * ic2 - u0 * ic2;
@@ -1209,18 +1287,20 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
             }
         }
     }
-    if (isFinal && a.snap) {
+    if constexpr (kFinal) {
+        if (a.snap) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            double v = res[k];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-            if (lane == 0) sRed[wave][k] = v;
-        }
-        __syncthreads();
-        if (tid < 5) {
-            double sum = 0.0;
-            for (int w = 0; w < kBlock / 64; ++w) sum += sRed[w][tid];
-            a.resPartial[static_cast<int64_t>(blockIdx.x) * 5 + tid] = sum;
+            for (int k = 0; k < 5; ++k) {
+                double v = res[k];
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+                if (lane == 0) sRed[wave][k] = v;
+            }
+            __syncthreads();
+            if (tid < 5) {
+                double sum = 0.0;
+                for (int w = 0; w < kBlock / 64; ++w) sum += sRed[w][tid];
+                a.resPartial[static_cast<int64_t>(blockIdx.x) * 5 + tid] = sum;
+            }
         }
     }
 }
