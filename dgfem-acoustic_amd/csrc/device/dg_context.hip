@@ -33,7 +33,6 @@ struct TableSource {
 struct dg_context;
 using StageFn = void (*)(dg_context*, int stage, const void* qin, void* qout, void* uout, bool snap);
 using PerStepFn = void (*)(dg_context*);
-using OccFn = int (*)();
 using ClosureFn = void (*)(dg_context*, const void* qin);
 using TraceFn = void (*)(dg_context*, int64_t n, const int64_t* elem, const int32_t* face, double* out,
                          double* normals);
@@ -86,9 +85,6 @@ struct dg_context {
     // dispatch
     StageFn stageFn = nullptr;
     int eb = 1;              // elements per block (batch) of the chosen stage kernel
-    OccFn occFn = nullptr;   // resident blocks per CU of a persistent stage kernel (else null):
-                             // FP64 (persistent blocks stage the operator table once; measured
-                             // faster), FP32 runs one batch per block (measured faster)
     bool mfma = false;       // matrix-core stage kernel in use
     void* afrag = nullptr;   // A-operand fragments of the matrix-core kernel
     std::vector<double> afragHost;
@@ -286,12 +282,12 @@ void buildAfragLane(dg_context* c, const dg::RefElement& R) {
     using M = dgk::MfmaShape<DIM, P>;
     using B = dgk::BlockMmShape<DIM, P>;
     buildAfrag<DIM, P>(c, R);
-    std::vector<double> lanes(static_cast<size_t>(64) * B::kAPerLane, 0.0);
-    for (int lane = 0; lane < 64; ++lane)
-        for (int mt = 0; mt < M::MT; ++mt)
+    // [row tile][lane][k-step (padded to KSP)]
+    std::vector<double> lanes(static_cast<size_t>(M::MT) * 64 * B::KSP, 0.0);
+    for (int mt = 0; mt < M::MT; ++mt)
+        for (int lane = 0; lane < 64; ++lane)
             for (int ks = 0; ks < M::KS; ++ks)
-                lanes[static_cast<size_t>(lane) * B::kAPerLane + mt * M::KS + ks] =
-                    c->afragHost[(mt * M::KS + ks) * 64 + lane];
+                lanes[(static_cast<size_t>(mt) * 64 + lane) * B::KSP + ks] = c->afragHost[(mt * M::KS + ks) * 64 + lane];
     c->afragHost.swap(lanes);
 }
 
@@ -353,25 +349,6 @@ void launchPost(dg_context* c) {
     p.computeResiduals = c->curResid;
     hipLaunchKernelGGL((dgk::postKernel<T, COMP>), dim3(static_cast<unsigned>(c->postBlocks)), dim3(256), 0, c->stream,
                        p, sourceArgs(c), static_cast<T*>(c->u), static_cast<T*>(COMP ? c->ulo : nullptr));
-}
-
-template <int DIM, int P, int NF, typename T, bool COMP, int MODE>
-int modeOccupancy() {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dgk::stageKernel<DIM, P, NF, T, COMP, MODE>, dgk::kBlock,
-                                                     0) != hipSuccess)
-        n = 0;
-    return n;
-}
-
-// Resident blocks per CU that every mode of the stage kernel reaches.
-template <int DIM, int P, int NF, typename T, bool COMP>
-int stageOccupancy() {
-    return std::min({modeOccupancy<DIM, P, NF, T, COMP, dgk::kModeFirst>(),
-                     modeOccupancy<DIM, P, NF, T, COMP, dgk::kModeMid>(),
-                     modeOccupancy<DIM, P, NF, T, COMP, dgk::kModeFinal>(),
-                     modeOccupancy<DIM, P, NF, T, COMP, dgk::kModeEuler>(),
-                     modeOccupancy<DIM, P, NF, T, COMP, dgk::kModeRHS>()});
 }
 
 template <int P, int NF, typename T>
@@ -453,7 +430,6 @@ bool pick(dg_context* c) {
     c->mfma = false;
     c->eb = dgk::Sizes<DIM, P>::EB;
     if (c->precision == DG_FP64) {
-        c->occFn = &stageOccupancy<DIM, P, NF, double, false>;
         c->stageFn = &launchStage<DIM, P, NF, double, false>;
         c->sourceFn = &launchSource<double, false>;
         c->postFn = &launchPost<double, false>;
@@ -805,15 +781,7 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         c->device = c->opt.device;
         HIPCHK(hipSetDevice(c->device));
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        c->nblocks = (c->K + c->eb - 1) / c->eb;
-        if (c->occFn) {
-            // persistent stage kernel: one block per resident slot (a multiple of the 8 XCDs)
-            int cus = 0;
-            HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-            const int64_t slots = static_cast<int64_t>(cus) * std::max(1, c->occFn());
-            if (slots < c->nblocks) c->nblocks = slots;
-            if (c->nblocks >= 8) c->nblocks -= c->nblocks % 8;
-        }
+        c->nblocks = (c->K + c->eb - 1) / c->eb;  // one batch of eb elements per block
         const size_t nstate = static_cast<size_t>(c->K) * c->Np * c->nf;
         auto alloc = [&](size_t n) -> void* {
             return c->precision == DG_FP64 ? static_cast<void*>(devAlloc<double>(c, n))

@@ -255,9 +255,6 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
     constexpr int JJ = DIM * DIM;
     constexpr int TS = static_cast<int>(sizeof(T));
     constexpr bool kOpsInLds = (L::kTotal * TS) <= 40 * 1024;
-    // FP64: persistent blocks loop over batches (measured faster); FP32: one batch per block
-    // (higher occupancy; measured faster). The host sizes the grid accordingly.
-    constexpr bool kPersistent = sizeof(T) == 8;
     constexpr bool kNeedU = MODE == kModeMid || MODE == kModeFinal;
     constexpr bool kFinal = MODE == kModeFinal || MODE == kModeEuler;
     // per-thread load counts
@@ -289,18 +286,10 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
     const T* ops = kOpsInLds ? sOps : a.ops;
     double res[5] = {0, 0, 0, 0, 0};
 
-    // batches of EB elements: each XCD takes a contiguous range, strided over its blocks (one
-    // batch per block when the grid covers all batches; persistent blocks otherwise)
-    const int64_t nBatches = (a.K + EB - 1) / EB;
-    int64_t bFirst = blockIdx.x, bEnd = nBatches, bStep = gridDim.x;
-    if (gridDim.x % 8 == 0) {
-        const int64_t xcd = blockIdx.x % 8, perXcd = gridDim.x / 8;
-        bFirst = nBatches * xcd / 8 + blockIdx.x / 8;
-        bEnd = nBatches * (xcd + 1) / 8;
-        bStep = perXcd;
-    }
-
-    auto batch = [&](int64_t bi, bool withTables) {
+    // one batch of EB elements per block (one batch per block measured faster than persistent
+    // blocks in both precisions); each XCD takes a contiguous range of batches
+    const int64_t bi = xcdRemap(blockIdx.x, gridDim.x);
+    {
     const int64_t e0 = bi * EB;
     const int nE = static_cast<int>((a.K - e0) < EB ? (a.K - e0) : EB);
     const bool active = tid < nE * NP;
@@ -311,7 +300,7 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
     U4 rOps[kOpsIt > 0 ? kOpsIt : 1];
     T rOpsTail = T(0);
     int32_t rI[kIIt];
-    if (withTables) {
+    {
 #pragma unroll
         for (int c = 0; c < kOpsIt; ++c) {
             const int x = tid + c * kBlock;
@@ -360,7 +349,7 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
     if constexpr (COMP && kFinal) loadNodeNT<NF>(a.ulo + ldBase, plo);
 
     // ---- LDS ----
-    if (withTables) {
+    {
 #pragma unroll
         for (int c = 0; c < kOpsIt; ++c) {
             const int x = tid + c * kBlock;
@@ -576,16 +565,6 @@ __global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void st
             }
         }
     }
-    };
-    if constexpr (kPersistent) {
-        bool first = true;
-        for (int64_t bi = bFirst; bi < bEnd; bi += bStep) {
-            batch(bi, first);
-            first = false;
-            __syncthreads();  // the next batch overwrites the staged state and fluxes
-        }
-    } else {
-        if (bFirst < bEnd) batch(bFirst, true);
     }
     if constexpr (kFinal) {
         if (a.snap) {
@@ -947,16 +926,23 @@ __global__ __launch_bounds__(kBlock) void stageKernelWave(
 //     per node, and the scaled numerical fluxes Fscale * pi once per face node, written as
 //     the B operand in matrix-core fragment order ([k-step][column tile][lane]; conflict-free);
 //  3. matrix core: rhs = [Dw_1 | .. | Dw_DIM | -LIFT] . B with the A fragments resident in
-//     registers, four independent accumulation chains per wave;
-//  4. the wave's result is transposed through its own (now dead) slice of B and the RK
-//     update is done node-major with 16-byte accesses, as in stageKernel.
+//     registers (one row tile per wave where the row tiles divide the 4 waves), two
+//     accumulation chains per tile;
+//  4. the results go to LDS node-major and the RK update is done with 16-byte accesses, as in
+//     stageKernel.
 template <int DIM, int P>
 struct BlockMmShape {
     using M = MfmaShape<DIM, P>;
     static constexpr int EB = 16;                        // elements per block
     static constexpr int KROWS = 4 * M::KS;              // B rows (volume, then lift)
     static constexpr int kBFloats = M::KS * 4 * 64;      // B fragments of the 4 column tiles
-    static constexpr int kAPerLane = (M::MT * M::KS + 3) / 4 * 4;  // A values per lane (padded)
+    static constexpr int KSP = (M::KS + 3) / 4 * 4;      // A values per lane and row tile (padded)
+    // MT in {1, 2, 4}: each wave takes one row tile and MT column tiles (A fragments of one row
+    // tile per lane); otherwise every row tile of one column tile
+    static constexpr bool kSplit = (4 % M::MT) == 0;
+    static constexpr int kMtW = kSplit ? 1 : M::MT;      // row tiles per wave
+    static constexpr int kCtW = kSplit ? M::MT : 1;      // column tiles per wave
+    static constexpr int kAPerLane = kMtW * KSP;
     static constexpr bool kSupported = M::kSupported && Sizes<DIM, P>::NP <= KROWS;
 };
 
@@ -968,7 +954,6 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
     using B = BlockMmShape<DIM, P>;
     using MT_ = MfmaTraits<T>;
     using V4 = typename Vec4<T>::V;
-    typedef uint32_t U4 __attribute__((ext_vector_type(4)));
     constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC, NFT = S::NFT, NF = 4, EB = B::EB;
     constexpr int JJ = DIM * DIM;
     constexpr int LIFT0 = 4 * M::KSV;  // first B row of the lift part
@@ -985,8 +970,10 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
     constexpr int padV = LIFT0 - M::KV, padL = B::KROWS - LIFT0 - NFT;
     constexpr int kPadIt = ((padV + padL) * 64 + kBlock - 1) / kBlock;
 
+    constexpr int QS = ldsStride(NP * NF);  // padded per-element stride (LDS banks)
     __shared__ __align__(32) T sB[B::kBFloats];
-    __shared__ __align__(32) T sQ[EB * NP * NF];
+    __shared__ __align__(32) T sQ[EB * QS];
+    __shared__ __align__(32) T sR[EB * QS];  // right-hand sides, node-major
     __shared__ T sJ[EB * JJ];
     __shared__ __align__(32) T sN[EB * NFC * 4];
     __shared__ int32_t sNb[EB * NFC];
@@ -1052,7 +1039,10 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
 #pragma unroll
     for (int c = 0; c < kQIt; ++c) {
         const int x = tid + c * kBlock;
-        if (x < nE * NP) storeNode<NF>(&sQ[x * NF], rq[c]);
+        if (x < nE * NP) {
+            const int e = x / NP;
+            storeNode<NF>(&sQ[e * QS + (x - e * NP) * NF], rq[c]);
+        }
     }
 #pragma unroll
     for (int c = 0; c < kJIt; ++c) {
@@ -1115,18 +1105,22 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
             if constexpr (COMP) loadNodeNT<NF>((a.snaplo ? a.snaplo : a.qin) + uNode[c] * NF, psl[c]);
         }
     }
-    // A fragments of this lane (constant operators), stored lane-contiguous ([lane][mt][ks]) so
-    // that they load as 16-byte vectors; issued here so that they arrive with the traces
+    // A fragments of this lane and the wave's row tiles (constant operators), stored
+    // [row tile][lane][k-step] so that they load as 16-byte vectors; issued here so that they
+    // arrive with the traces
+    const int mt0 = B::kSplit ? wave % M::MT : 0;                 // first row tile of the wave
+    const int ct0 = B::kSplit ? (wave / M::MT) * M::MT : wave;    // first column tile of the wave
     T af[B::kAPerLane];
-    {
-        const V4* src = reinterpret_cast<const V4*>(afrag + lane * B::kAPerLane);
 #pragma unroll
-        for (int x = 0; x < B::kAPerLane / 4; ++x) {
+    for (int m = 0; m < B::kMtW; ++m) {
+        const V4* src = reinterpret_cast<const V4*>(afrag + (static_cast<int64_t>(mt0 + m) * 64 + lane) * B::KSP);
+#pragma unroll
+        for (int x = 0; x < B::KSP / 4; ++x) {
             const V4 v = src[x];
-            af[4 * x] = v.x;
-            af[4 * x + 1] = v.y;
-            af[4 * x + 2] = v.z;
-            af[4 * x + 3] = v.w;
+            af[m * B::KSP + 4 * x] = v.x;
+            af[m * B::KSP + 4 * x + 1] = v.y;
+            af[m * B::KSP + 4 * x + 2] = v.z;
+            af[m * B::KSP + 4 * x + 3] = v.w;
         }
     }
 
@@ -1174,7 +1168,7 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
             const int own = sI[L::kMask + f * NFP + an];
             const int nb = sNb[ef];
             T qm[NF], pi[NF], out[NF];
-            loadNode<NF>(&sQ[(e * NP + own) * NF], qm);
+            loadNode<NF>(&sQ[e * QS + own * NF], qm);
             numericalFlux<DIM, NF>(a.ph, nv, nb <= -3 ? 0 : nb, qm, qp[c], pi);
 #pragma unroll
             for (int k = 0; k < NF; ++k) out[k] = fs * pi[k];
@@ -1183,30 +1177,34 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
     }
     __syncthreads();
 
-    // ---- 3. matrix core: this wave's column tile (elements 4*wave .. 4*wave+3) ----
-    typename MT_::V acc[M::MT][2];
+    // ---- 3. matrix core: the wave's row tiles x column tiles (4 elements x 4 fields each) ----
+    typename MT_::V acc[B::kMtW][B::kCtW][2];
 #pragma unroll
-    for (int mt = 0; mt < M::MT; ++mt) acc[mt][0] = acc[mt][1] = typename MT_::V{T(0), T(0), T(0), T(0)};
-    T* Bw = sB + wave * 64 + lane;
+    for (int m = 0; m < B::kMtW; ++m)
+#pragma unroll
+        for (int ct = 0; ct < B::kCtW; ++ct) acc[m][ct][0] = acc[m][ct][1] = typename MT_::V{T(0), T(0), T(0), T(0)};
 #pragma unroll
     for (int ks = 0; ks < M::KS; ++ks) {
-        const T b = Bw[ks * 256];
 #pragma unroll
-        for (int mt = 0; mt < M::MT; ++mt) acc[mt][ks & 1] = MT_::mma(af[mt * M::KS + ks], b, acc[mt][ks & 1]);
-    }
-    waveSync();
-    // 4a. transpose through this wave's own slice of B: element row i, column lane%16
+        for (int ct = 0; ct < B::kCtW; ++ct) {
+            const T b = sB[(ks * 4 + ct0 + ct) * 64 + lane];
 #pragma unroll
-    for (int mt = 0; mt < M::MT; ++mt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = 16 * mt + MT_::row(lane, r);
-            if (row < NP) {
-                const T v = acc[mt][0][r] + acc[mt][1][r];
-                Bw[((row >> 2) * 4) * 64 + (row & 3) * 16 - lane + (lane & 15)] = v;
-            }
+            for (int m = 0; m < B::kMtW; ++m)
+                acc[m][ct][ks & 1] = MT_::mma(af[m * B::KSP + ks], b, acc[m][ct][ks & 1]);
         }
-    waveSync();
+    }
+    // 4a. results node-major: row = node, column lane%16 = 4 * element + field within the tile
+#pragma unroll
+    for (int m = 0; m < B::kMtW; ++m)
+#pragma unroll
+        for (int ct = 0; ct < B::kCtW; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * (mt0 + m) + MT_::row(lane, r);
+                const int col = lane & 15;
+                if (row < NP) sR[(4 * (ct0 + ct) + (col >> 2)) * QS + row * NF + (col & 3)] = acc[m][ct][0][r] + acc[m][ct][1][r];
+            }
+    __syncthreads();
 
     // ---- 4b. RK update, node-major over this wave's 4 elements ----
     double res[5] = {0, 0, 0, 0, 0};
@@ -1217,8 +1215,8 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
         const int el = it / NP, i = it - el * NP;
         const int eb = 4 * wave + el;
         T rhs[NF], q0[NF];
-        loadNode<NF>(&sB[((i >> 2) * 4 + wave) * 64 + (i & 3) * 16 + el * 4], rhs);
-        loadNode<NF>(&sQ[(eb * NP + i) * NF], q0);
+        loadNode<NF>(&sR[eb * QS + i * NF], rhs);
+        loadNode<NF>(&sQ[eb * QS + i * NF], q0);
         const int64_t base = uNode[c] * NF;
         const T dt = a.dt;
         if constexpr (MODE == kModeRHS) {
