@@ -1336,16 +1336,31 @@ __device__ inline void applySources(const SourceArgs& s, int64_t slot, T* __rest
         }
         if (!active) continue;
         const int64_t b = s.nodeOffsets[src], e = s.nodeOffsets[src + 1];
-        for (int64_t x = b + threadIdx.x; x < e; x += blockDim.x) {
-            const int64_t base = s.nodeIdx[x] * s.nf;
-            const T hi = static_cast<T>(value);
-            u[base] = hi;
-            if constexpr (COMP) ulo[base] = static_cast<T>(value - static_cast<double>(hi));
-            if (s.nf == 5) {
-                const double rv = value * s.ic2;
-                const T rh = static_cast<T>(rv);
-                u[base + 4] = rh;
-                if constexpr (COMP) ulo[base + 4] = static_cast<T>(rv - static_cast<double>(rh));
+        const T hi = static_cast<T>(value);
+        const T lo = static_cast<T>(value - static_cast<double>(hi));
+        const double rv = value * s.ic2;
+        const T rh = static_cast<T>(rv);
+        const T rl = static_cast<T>(rv - static_cast<double>(rh));
+        // node indices of kU entries per thread are loaded (from clamped addresses) before any
+        // store, so one round trip covers kU * blockDim entries
+        constexpr int kU = 8;
+        for (int64_t x0 = b + threadIdx.x; x0 < e; x0 += kU * blockDim.x) {
+            int64_t idx[kU];
+#pragma unroll
+            for (int q = 0; q < kU; ++q) {
+                const int64_t x = x0 + static_cast<int64_t>(q) * blockDim.x;
+                idx[q] = s.nodeIdx[x < e ? x : e - 1];
+            }
+#pragma unroll
+            for (int q = 0; q < kU; ++q) {
+                if (x0 + static_cast<int64_t>(q) * blockDim.x >= e) break;
+                const int64_t base = idx[q] * s.nf;
+                u[base] = hi;
+                if constexpr (COMP) ulo[base] = lo;
+                if (s.nf == 5) {
+                    u[base + 4] = rh;
+                    if constexpr (COMP) ulo[base + 4] = rl;
+                }
             }
         }
     }
