@@ -164,7 +164,7 @@ void launchStage(dg_context* c, int stage, const void* qin, void* qout, void* uo
     a.dt = static_cast<T>(c->curDt);
     a.stage = stage;
     a.ph = makePhysics<T>(c->phys);
-    const dim3 grid(static_cast<unsigned>(c->nblocks)), block(dgk::kBlock);
+    const dim3 grid(static_cast<unsigned>(c->nblocks)), block(dgk::StageWaves<DIM, P, NF, T>::kBlockSize);
     switch (dgk::stageMode(stage)) {
         case dgk::kModeFirst:
             hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeFirst>), grid, block, 0, c->stream, a);
@@ -428,12 +428,13 @@ bool pick(dg_context* c) {
         }
     }
     c->mfma = false;
-    c->eb = dgk::Sizes<DIM, P>::EB;
     if (c->precision == DG_FP64) {
+        c->eb = dgk::StageWaves<DIM, P, NF, double>::EB;
         c->stageFn = &launchStage<DIM, P, NF, double, false>;
         c->sourceFn = &launchSource<double, false>;
         c->postFn = &launchPost<double, false>;
     } else {
+        c->eb = dgk::StageWaves<DIM, P, NF, float>::EB;
         c->stageFn = &launchStage<DIM, P, NF, float, true>;
         c->sourceFn = &launchSource<float, true>;
         c->postFn = &launchPost<float, true>;

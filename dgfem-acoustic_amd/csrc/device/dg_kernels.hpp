@@ -223,7 +223,12 @@ __device__ inline void twoSum(T a, T b, T& s, T& e) {
 template <int DIM, int P, int NF, typename T>
 struct StageWaves {
     static constexpr int NP = Sizes<DIM, P>::NP;
-    static constexpr int value = (sizeof(T) == 4 && NF == 4 && NP >= 10 && NP <= 20) ? 7 : 1;
+    static constexpr bool kBig = sizeof(T) == 4 && NF == 4 && NP >= 10 && NP <= 20;
+    // FP32, 4 fields, 10..20 nodes: 512-thread blocks (one operator table per 2x the elements)
+    // fit 4 blocks = 8 waves per SIMD in LDS; the kernel is asked to stay within that
+    static constexpr int kBlockSize = kBig ? 512 : kBlock;
+    static constexpr int value = kBig ? 8 : 1;
+    static constexpr int EB = kBlockSize / NP;  // elements per block
 };
 
 // Padded LDS row stride (in elements) for per-element blocks of n values: == 4 (mod 16).
@@ -248,10 +253,11 @@ __host__ __device__ constexpr int stageMode(int stage) {
 // RK operands consumed at the end) -> LDS -> barrier; phase 2 (the neighbour traces of all of the
 // thread's face nodes) -> fluxes -> barrier; then the contractions and the RK update.
 template <int DIM, int P, int NF, typename T, bool COMP, int MODE>
-__global__ __launch_bounds__(kBlock, (StageWaves<DIM, P, NF, T>::value)) void stageKernel(StageArgs<T> a) {
+__global__ __launch_bounds__((StageWaves<DIM, P, NF, T>::kBlockSize), (StageWaves<DIM, P, NF, T>::value)) void stageKernel(StageArgs<T> a) {
     using S = Sizes<DIM, P>;
     using L = OpLayout<DIM, P>;
-    constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC, NFT = S::NFT, EB = S::EB;
+    constexpr int kBlock = StageWaves<DIM, P, NF, T>::kBlockSize;  // threads per block
+    constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC, NFT = S::NFT, EB = StageWaves<DIM, P, NF, T>::EB;
     constexpr int JJ = DIM * DIM;
     constexpr int TS = static_cast<int>(sizeof(T));
     constexpr bool kOpsInLds = (L::kTotal * TS) <= 40 * 1024;
