@@ -223,11 +223,12 @@ __device__ inline void twoSum(T a, T b, T& s, T& e) {
 template <int DIM, int P, int NF, typename T>
 struct StageWaves {
     static constexpr int NP = Sizes<DIM, P>::NP;
-    static constexpr bool kBig = sizeof(T) == 4 && NF == 4 && NP >= 10 && NP <= 20;
-    // FP32, 4 fields, 10..20 nodes: 512-thread blocks (one operator table per 2x the elements)
-    // fit 4 blocks = 8 waves per SIMD in LDS; the kernel is asked to stay within that
+    static constexpr bool kBig = NF == 4 && NP >= 10 && NP <= 20;
+    // 4 fields, 10..20 nodes: 512-thread blocks (one operator table per 2x the elements) fit
+    // 4 blocks = 8 waves per SIMD in LDS in FP32, 2 blocks = 4 waves per SIMD in FP64; the
+    // kernel is asked to stay within the registers that allows
     static constexpr int kBlockSize = kBig ? 512 : kBlock;
-    static constexpr int value = kBig ? 8 : 1;
+    static constexpr int value = kBig ? (sizeof(T) == 4 ? 8 : 4) : 1;
     static constexpr int EB = kBlockSize / NP;  // elements per block
 };
 
