@@ -54,38 +54,45 @@ struct Scope {
 
 constexpr int kPer = 32;  // samples per gradient block
 
-template <int NFP>
+template <int NIN_, int NOUT_>
 struct Net {
-    static constexpr int NIN = 2 * NFP + 2, NOUT = NFP;
+    static constexpr int NIN = NIN_, NOUT = NOUT_;
     static constexpr int P = dgc::numParams(NIN, NOUT);
 };
 
-// Face-node count of a network with these sizes, or -1.
-int netNfp(int nin, int nout) {
-    for (int nfp = 2; nfp <= 6; ++nfp)
-        if (nin == 2 * nfp + 2 && nout == nfp) return nfp;
+// Network shapes built: 2D faces of order 1..5 (n_in = 2 Nfp + 2, Nfp = p + 1) and 3D faces
+// of order 1..3 (n_in = 3 Nfp + 2, Nfp = 3, 6, 10). Index into the list, or -1.
+constexpr int kShapes[][2] = {{6, 2}, {8, 3}, {10, 4}, {12, 5}, {14, 6}, {11, 3}, {20, 6}, {32, 10}};
+int netShape(int nin, int nout) {
+    for (int i = 0; i < static_cast<int>(sizeof(kShapes) / sizeof(kShapes[0])); ++i)
+        if (kShapes[i][0] == nin && kShapes[i][1] == nout) return i;
     return -1;
 }
 
-#define DG_NET_DISPATCH(nfp, FN, ...)     \
-    switch (nfp) {                        \
-        case 2: FN<2>(__VA_ARGS__); break; \
-        case 3: FN<3>(__VA_ARGS__); break; \
-        case 4: FN<4>(__VA_ARGS__); break; \
-        case 5: FN<5>(__VA_ARGS__); break; \
-        case 6: FN<6>(__VA_ARGS__); break; \
-        default: break;                   \
+#define DG_NET_DISPATCH(shape, FN, ...)          \
+    switch (shape) {                             \
+        case 0: FN<6, 2>(__VA_ARGS__); break;    \
+        case 1: FN<8, 3>(__VA_ARGS__); break;    \
+        case 2: FN<10, 4>(__VA_ARGS__); break;   \
+        case 3: FN<12, 5>(__VA_ARGS__); break;   \
+        case 4: FN<14, 6>(__VA_ARGS__); break;   \
+        case 5: FN<11, 3>(__VA_ARGS__); break;   \
+        case 6: FN<20, 6>(__VA_ARGS__); break;   \
+        case 7: FN<32, 10>(__VA_ARGS__); break;  \
+        default: break;                          \
     }
 
-const char* kSizeError = "closure sizes must be n_in = 2 Nfp + 2, n_out = Nfp with Nfp in 2..6";
+const char* kSizeError =
+    "closure sizes must be n_in = 2 Nfp + 2 with Nfp in 2..6 (2D) or n_in = 3 Nfp + 2 with Nfp in {3, 6, 10} "
+    "(3D), n_out = Nfp";
 
 // Loss (sum of squares, unscaled) and block partial gradients of samples
 // [first, first + count) in the order of idx, then their (scaled) reduction.
-template <int NFP>
+template <int NIN, int NOUT>
 void lossGrad(Scope& s, const uint16_t* w, const float* X, const float* Y, const int64_t* idx, int64_t first,
                 int64_t count, float* gpart, double* lpart, float* grad, float* master, float* m1, float* m2,
                 uint16_t* wbf, const dg_train_params* prm, int64_t step, bool update, double* lossHost) {
-    using N = Net<NFP>;
+    using N = Net<NIN, NOUT>;
     const int nparts = static_cast<int>((count + kPer - 1) / kPer);
     const float scale = 1.0f / (static_cast<float>(count) * N::NOUT);
     hipLaunchKernelGGL((dgc::gradKernel<N::NIN, N::NOUT>), dim3(nparts), dim3(64), 0, s.stream, w, X, Y, idx, first,
@@ -119,9 +126,9 @@ uint64_t splitmix(uint64_t& st) {
     return z ^ (z >> 31);
 }
 
-template <int NFP>
+template <int NIN, int NOUT>
 void forwardImpl(const uint16_t* params, const float* X, int64_t n, float* Y, int device) {
-    using N = Net<NFP>;
+    using N = Net<NIN, NOUT>;
     Scope s(device);
     const uint16_t* w = s.upload(params, N::P);
     const float* dx = s.upload(X, static_cast<size_t>(n) * N::NIN);
@@ -136,10 +143,10 @@ void forwardImpl(const uint16_t* params, const float* X, int64_t n, float* Y, in
     CHK(hipStreamSynchronize(s.stream));
 }
 
-template <int NFP>
+template <int NIN, int NOUT>
 void lossGradImpl(const uint16_t* params, const float* X, const float* Y, int64_t n, double* loss, float* grad,
                   int device) {
-    using N = Net<NFP>;
+    using N = Net<NIN, NOUT>;
     Scope s(device);
     const uint16_t* w = s.upload(params, N::P);
     const float* dx = s.upload(X, static_cast<size_t>(n) * N::NIN);
@@ -149,17 +156,17 @@ void lossGradImpl(const uint16_t* params, const float* X, const float* Y, int64_
     double* lpart = s.alloc<double>(nparts);
     float* dg = s.alloc<float>(N::P);
     double lsum = 0;
-    lossGrad<NFP>(s, w, dx, dy, nullptr, 0, n, gpart, lpart, dg, nullptr, nullptr, nullptr, nullptr, nullptr, 0, false,
-                  &lsum);
+    lossGrad<NIN, NOUT>(s, w, dx, dy, nullptr, 0, n, gpart, lpart, dg, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+                        false, &lsum);
     *loss = lsum / (static_cast<double>(n) * N::NOUT);
     if (grad) CHK(hipMemcpyAsync(grad, dg, N::P * sizeof(float), hipMemcpyDeviceToHost, s.stream));
     CHK(hipStreamSynchronize(s.stream));
 }
 
-template <int NFP>
+template <int NIN, int NOUT>
 void trainImpl(float* master, uint16_t* params, const float* X, const float* Y, int64_t n, const dg_train_params* prm,
                double* epochLoss, int device) {
-    using N = Net<NFP>;
+    using N = Net<NIN, NOUT>;
     Scope s(device);
     std::vector<uint16_t> wb(N::P);
     for (int i = 0; i < N::P; ++i) wb[i] = dgc::floatToBf16(master[i]);
@@ -191,8 +198,8 @@ void trainImpl(float* master, uint16_t* params, const float* X, const float* Y, 
             const int64_t first = b * batch;
             const int64_t count = (first + batch <= n) ? batch : n - first;
             ++step;
-            lossGrad<NFP>(s, w, dx, dy, didx, first, count, gpart, lpart + b * nparts, nullptr, dm, m1, m2, w, prm, step,
-                          true, nullptr);
+            lossGrad<NIN, NOUT>(s, w, dx, dy, didx, first, count, gpart, lpart + b * nparts, nullptr, dm, m1, m2, w, prm,
+                                step, true, nullptr);
         }
         // blocks of a short last batch leave their slots of lpart as written in earlier epochs:
         // only the first ceil(count/32) partials of each batch are summed
@@ -220,7 +227,7 @@ extern "C" {
 DG_API const char* dg_closure_last_error(void) { return g_closureError.c_str(); }
 
 DG_API int64_t dg_closure_num_params(int32_t n_in, int32_t n_out) {
-    return netNfp(n_in, n_out) < 0 ? -1 : dgc::numParams(n_in, n_out);
+    return netShape(n_in, n_out) < 0 ? -1 : dgc::numParams(n_in, n_out);
 }
 
 DG_API void dg_closure_to_bf16(const float* x, int64_t n, uint16_t* out) {
@@ -230,13 +237,13 @@ DG_API void dg_closure_to_bf16(const float* x, int64_t n, uint16_t* out) {
 DG_API dg_status dg_closure_forward(int32_t n_in, int32_t n_out, const uint16_t* params, const float* X, int64_t n,
                                     float* Y, int32_t device) {
     if (!params || !X || !Y || n < 0) return DG_ERR_INVALID;
-    const int nfp = netNfp(n_in, n_out);
-    if (nfp < 0) {
+    const int shape = netShape(n_in, n_out);
+    if (shape < 0) {
         g_closureError = kSizeError;
         return DG_ERR_UNSUPPORTED;
     }
     try {
-        DG_NET_DISPATCH(nfp, forwardImpl, params, X, n, Y, device)
+        DG_NET_DISPATCH(shape, forwardImpl, params, X, n, Y, device)
     } catch (const std::exception& ex) {
         g_closureError = ex.what();
         return DG_ERR_DEVICE;
@@ -247,13 +254,13 @@ DG_API dg_status dg_closure_forward(int32_t n_in, int32_t n_out, const uint16_t*
 DG_API dg_status dg_closure_loss_grad(int32_t n_in, int32_t n_out, const uint16_t* params, const float* X,
                                       const float* Y, int64_t n, double* loss, float* grad, int32_t device) {
     if (!params || !X || !Y || n <= 0 || !loss) return DG_ERR_INVALID;
-    const int nfp = netNfp(n_in, n_out);
-    if (nfp < 0) {
+    const int shape = netShape(n_in, n_out);
+    if (shape < 0) {
         g_closureError = kSizeError;
         return DG_ERR_UNSUPPORTED;
     }
     try {
-        DG_NET_DISPATCH(nfp, lossGradImpl, params, X, Y, n, loss, grad, device)
+        DG_NET_DISPATCH(shape, lossGradImpl, params, X, Y, n, loss, grad, device)
     } catch (const std::exception& ex) {
         g_closureError = ex.what();
         return DG_ERR_DEVICE;
@@ -265,52 +272,87 @@ DG_API dg_status dg_closure_train(int32_t n_in, int32_t n_out, float* master, ui
                                   const float* Y, int64_t n, const dg_train_params* prm, double* epoch_loss,
                                   int32_t device) {
     if (!master || !params || !X || !Y || n <= 0 || !prm || prm->epochs < 0 || prm->batch <= 0) return DG_ERR_INVALID;
-    const int nfp = netNfp(n_in, n_out);
-    if (nfp < 0) {
+    const int shape = netShape(n_in, n_out);
+    if (shape < 0) {
         g_closureError = kSizeError;
         return DG_ERR_UNSUPPORTED;
     }
     try {
-        DG_NET_DISPATCH(nfp, trainImpl, master, params, X, Y, n, prm, epoch_loss, device)
+        DG_NET_DISPATCH(shape, trainImpl, master, params, X, Y, n, prm, epoch_loss, device)
     } catch (const std::exception& ex) {
         g_closureError = ex.what();
         return DG_ERR_DEVICE;
     }
     return DG_OK;
 }
-DG_API dg_status dg_closure_features(int32_t nfp, int32_t nf, int64_t n, const double* traces, const double* normals,
-                                     const dg_physics* ph, float* X, float* Y, float* S) {
-    if (nfp < 2 || nfp > 6 || (nf != 4 && nf != 5) || n < 0 || !traces || !normals || !ph || !X) return DG_ERR_INVALID;
-    const int nin = 2 * nfp + 2, nout = nfp;
+DG_API dg_status dg_closure_features(int32_t dim, int32_t nfp, int32_t nf, int64_t n, const double* traces,
+                                     const double* normals, const dg_physics* ph, float* X, float* Y, float* S) {
+    if ((dim != 2 && dim != 3) || nfp < 2 || nfp > 10 || (nf != 4 && nf != 5) || n < 0 || !traces || !normals || !ph ||
+        !X)
+        return DG_ERR_INVALID;
+    const int nt = dim - 1, nin = dim * nfp + 2, nout = nfp;
     const double irc = 1.0 / (ph->rho0 * ph->c0);
-    std::vector<double> wo(nfp), tt(nfp);
+    std::vector<double> wo(nfp), tt(static_cast<size_t>(nt) * nfp);
     for (int64_t i = 0; i < n; ++i) {
         const double* nv = normals + 3 * i;
-        const double n0 = nv[0], n1 = nv[1], t0 = -n1, t1 = n0;
+        double nn[3] = {nv[0], nv[1], dim == 3 ? nv[2] : 0.0}, t[2][3] = {{0, 0, 0}, {0, 0, 0}};
+        double vn0 = 0.0, vt0 = 0.0;
+        for (int c = 0; c < dim; ++c) vn0 += ph->v0[c] * nn[c];
+        if (dim == 2) {
+            t[0][0] = -nn[1];
+            t[0][1] = nn[0];
+            vt0 = ph->v0[0] * t[0][0] + ph->v0[1] * t[0][1];
+        } else {
+            // t1 along the tangential mean flow, else along the axis least aligned with n
+            double u[3];
+            for (int c = 0; c < 3; ++c) u[c] = ph->v0[c] - vn0 * nn[c];
+            vt0 = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            if (!(vt0 > 1e-6 * ph->c0)) {
+                const double ax = std::fabs(nn[0]), ay = std::fabs(nn[1]), az = std::fabs(nn[2]);
+                const int k = (ax <= ay && ax <= az) ? 0 : (ay <= az ? 1 : 2);
+                for (int c = 0; c < 3; ++c) u[c] = (c == k ? 1.0 : 0.0) - nn[k] * nn[c];
+                vt0 = 0.0;
+            }
+            const double iu = 1.0 / std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            for (int c = 0; c < 3; ++c) t[0][c] = u[c] * iu;
+            t[1][0] = nn[1] * t[0][2] - nn[2] * t[0][1];
+            t[1][1] = nn[2] * t[0][0] - nn[0] * t[0][2];
+            t[1][2] = nn[0] * t[0][1] - nn[1] * t[0][0];
+        }
         float ss = 0.f;
         for (int j = 0; j < nfp; ++j) {
             const double* q = traces + ((i * 2) * nfp + j) * nf;
-            wo[j] = q[0] * irc + (q[1] * n0 + q[2] * n1);
-            tt[j] = q[1] * t0 + q[2] * t1;
-            const float a = static_cast<float>(wo[j]), b = static_cast<float>(tt[j]);
-            ss += a * a + b * b;
+            double vn = 0.0;
+            for (int c = 0; c < dim; ++c) vn += q[1 + c] * nn[c];
+            wo[j] = q[0] * irc + vn;
+            const float a = static_cast<float>(wo[j]);
+            ss += a * a;
+            for (int k = 0; k < nt; ++k) {
+                double v = 0.0;
+                for (int c = 0; c < dim; ++c) v += q[1 + c] * t[k][c];
+                tt[static_cast<size_t>(k) * nfp + j] = v;
+                const float b = static_cast<float>(v);
+                ss += b * b;
+            }
         }
         const float s = std::sqrt(ss / nfp) + 1e-30f;
         if (S) S[i] = s;
         float* x = X + i * nin;
         for (int j = 0; j < nfp; ++j) {
             x[j] = static_cast<float>(wo[j]) / s;
-            x[nfp + j] = static_cast<float>(tt[j]) / s;
+            for (int k = 0; k < nt; ++k)
+                x[(1 + k) * nfp + j] = static_cast<float>(tt[static_cast<size_t>(k) * nfp + j]) / s;
         }
-        x[2 * nfp] = static_cast<float>((ph->v0[0] * n0 + ph->v0[1] * n1) / ph->c0);
-        x[2 * nfp + 1] = static_cast<float>((ph->v0[0] * t0 + ph->v0[1] * t1) / ph->c0);
+        x[dim * nfp] = static_cast<float>(vn0 / ph->c0);
+        x[dim * nfp + 1] = static_cast<float>(vt0 / ph->c0);
         if (Y) {
             // target: the incoming characteristic of the neighbour's trace
             float* y = Y + i * nout;
             for (int j = 0; j < nfp; ++j) {
                 const double* qe = traces + ((i * 2 + 1) * nfp + j) * nf;
-                const double wie = qe[0] * irc - (qe[1] * n0 + qe[2] * n1);
-                y[j] = static_cast<float>(wie / s);
+                double vn = 0.0;
+                for (int c = 0; c < dim; ++c) vn += qe[1 + c] * nn[c];
+                y[j] = static_cast<float>((qe[0] * irc - vn) / s);
             }
         }
     }

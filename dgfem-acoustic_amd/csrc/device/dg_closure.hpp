@@ -4,22 +4,25 @@
 // On a truncation face the exterior state of the interior numerical flux
 // n.{F} + (c0/2)(q- - q+) is built so that the flux carries a prescribed incoming
 // characteristic predicted by a small MLP from the outgoing part of the interior trace.
-// 2D faces, face frame n (outward unit normal), t = (-n_y, n_x); per face node j:
-//   P = p / (rho0 c0), N = v.n, T = v.t; characteristics of the normal flux Jacobian:
-//   w_out = P + N (speed vn0 + c0), w_in = P - N (speed vn0 - c0), T (speed vn0)
-//   scale s = sqrt(mean_j (w_out^2 + T^2)) + 1e-30
-//   x = [w_out/s, T/s, vn0/c0, vt0/c0]                      (n_in = 2 Nfp + 2)
-//   y = W3 tanh(W2 tanh(W1 x + b1) + b2) + b3               (n_out = Nfp), w_hat = s y
+// Face frame: n (outward unit normal) and DIM-1 tangents; 2D t = (-n_y, n_x); 3D t1 along
+// the tangential mean flow v0 - vn0 n when |v0 - vn0 n| > 1e-6 c0 (else along e_a - (e_a.n) n
+// for the axis a least aligned with n) and t2 = n x t1. Per face node j:
+//   P = p / (rho0 c0), N = v.n, T_c = v.t_c; characteristics of the normal flux Jacobian:
+//   w_out = P + N (speed vn0 + c0), w_in = P - N (speed vn0 - c0), T_c (speed vn0)
+//   scale s = sqrt(mean_j (w_out^2 + sum_c T_c^2)) + 1e-30
+//   x = [w_out/s, T_1/s, (T_2/s,) vn0/c0, vt0/c0]  (n_in = DIM Nfp + 2; 2D vt0 = v0.t,
+//                                                  3D vt0 = |v0 - vn0 n|)
+//   y = W3 tanh(W2 tanh(W1 x + b1) + b2) + b3     (n_out = Nfp), w_hat = s y
 // Exterior state (upwind flux with incoming values w_hat and 0):
 //   w_out+ = w_out-
 //   w_in+  = (2 lam w_hat - (lam + c0) w_in-) / (lam - c0),  lam = vn0 - c0
-//   T+     = T- (vn0 >= 0);  -(vn0 + c0) T- / (vn0 - c0) (vn0 < 0: no incoming vorticity)
-//   p+ = rho0 c0 (w_out+ + w_in+)/2,  v+ = (w_out+ - w_in+)/2 n + T+ t,  vz+ = vz-,
+//   T_c+   = T_c- (vn0 >= 0);  -(vn0 + c0) T_c- / (vn0 - c0) (vn0 < 0: no incoming vorticity)
+//   p+ = rho0 c0 (w_out+ + w_in+)/2,  v+ = (w_out+ - w_in+)/2 n + sum_c T_c+ t_c (2D: vz+ = vz-),
 //   rho+ = rho- + (p+ - p-)/c0^2
 // The network never sees w_in-, so it cannot feed the interior's incoming content back in;
 // the zero network is the exact characteristic (first-order non-reflecting) condition. The
 // map is homogeneous of degree one in the trace and equivariant under rotations of the
-// face. Weights are BF16, activations and accumulation FP32. Parameter order (row-major,
+// face (in 3D when the tangential mean flow is nonzero; else the tangents follow the axes). Weights are BF16, activations and accumulation FP32. Parameter order (row-major,
 // out x in): W1[64][n_in], b1[64], W2[64][64], b2[64], W3[n_out][64], b3[n_out].
 #pragma once
 #include <hip/hip_runtime.h>
@@ -151,12 +154,13 @@ struct ClosureArgs {
 constexpr int kClosureFacesPerBlock = 4;
 
 // Exterior states of the learned faces for one stage: one wave per face, four per block.
-template <int P, int NF, typename T>
+template <int DIM, int P, int NF, typename T>
 __global__ __launch_bounds__(256) void closureKernel(ClosureArgs<T> a) {
-    using S = dgk::Sizes<2, P>;
-    using L = dgk::OpLayout<2, P>;
+    using S = dgk::Sizes<DIM, P>;
+    using L = dgk::OpLayout<DIM, P>;
     constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC;
-    constexpr int NIN = 2 * NFP + 2, NOUT = NFP;
+    constexpr int NT = DIM - 1;  // tangential directions
+    constexpr int NIN = DIM * NFP + 2, NOUT = NFP;
     using N = NetLds<NIN, NOUT>;
     __shared__ float net[N::kTotal];
     __shared__ float buf[kClosureFacesPerBlock][NIN + 2 * kH + NOUT + 2];
@@ -172,52 +176,89 @@ __global__ __launch_bounds__(256) void closureKernel(ClosureArgs<T> a) {
     const int64_t ef = a.faces[idx];
     const int64_t e = ef / NFC;
     const int f = static_cast<int>(ef - e * NFC);
-    // outward unit normal
-    const T n0 = a.fgeo[(e * NFC + f) * 4], n1 = a.fgeo[(e * NFC + f) * 4 + 1];
-    const T t0 = -n1, t1 = n0;
-    const T vn0 = a.ph.v0[0] * n0 + a.ph.v0[1] * n1;
-    const T vt0 = a.ph.v0[0] * t0 + a.ph.v0[1] * t1;
-    T qm[NF], wo = T(0), wi = T(0), tt = T(0);
+    const T c0 = a.ph.c0;
+    // face frame: outward unit normal n and tangents t[NT]
+    T n[3] = {a.fgeo[(e * NFC + f) * 4], a.fgeo[(e * NFC + f) * 4 + 1], T(0)};
+    T t[NT][3];
+    T vn0, vt0;
+    if constexpr (DIM == 2) {
+        t[0][0] = -n[1];
+        t[0][1] = n[0];
+        t[0][2] = T(0);
+        vn0 = a.ph.v0[0] * n[0] + a.ph.v0[1] * n[1];
+        vt0 = a.ph.v0[0] * t[0][0] + a.ph.v0[1] * t[0][1];
+    } else {
+        n[2] = a.fgeo[(e * NFC + f) * 4 + 2];
+        vn0 = a.ph.v0[0] * n[0] + a.ph.v0[1] * n[1] + a.ph.v0[2] * n[2];
+        // t1 along the tangential mean flow, else along the axis least aligned with n
+        T u[3];
+        for (int c = 0; c < 3; ++c) u[c] = a.ph.v0[c] - vn0 * n[c];
+        vt0 = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        if (!(vt0 > T(1e-6) * c0)) {
+            const T ax = fabs(n[0]), ay = fabs(n[1]), az = fabs(n[2]);
+            const int k = (ax <= ay && ax <= az) ? 0 : (ay <= az ? 1 : 2);
+            for (int c = 0; c < 3; ++c) u[c] = (c == k ? T(1) : T(0)) - n[k] * n[c];
+            vt0 = T(0);
+        }
+        const T iu = T(1) / sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        for (int c = 0; c < 3; ++c) t[0][c] = u[c] * iu;
+        t[1][0] = n[1] * t[0][2] - n[2] * t[0][1];
+        t[1][1] = n[2] * t[0][0] - n[0] * t[0][2];
+        t[1][2] = n[0] * t[0][1] - n[1] * t[0][0];
+    }
+    T qm[NF], wo = T(0), wi = T(0), tt[NT];
+    for (int c = 0; c < NT; ++c) tt[c] = T(0);
     if (lane < NFP) {
         const int node = a.iops[L::kMask + f * NFP + lane];
         const T* q = a.qin + (e * NP + node) * NF;
 #pragma unroll
         for (int k = 0; k < NF; ++k) qm[k] = q[k];
         const T Pv = qm[0] * a.ph.irc0;
-        const T Nn = qm[1] * n0 + qm[2] * n1;
+        T Nn = qm[1] * n[0] + qm[2] * n[1];
+        if constexpr (DIM == 3) Nn += qm[3] * n[2];
         wo = Pv + Nn;
         wi = Pv - Nn;
-        tt = qm[1] * t0 + qm[2] * t1;
+        for (int c = 0; c < NT; ++c) {
+            tt[c] = qm[1] * t[c][0] + qm[2] * t[c][1];
+            if constexpr (DIM == 3) tt[c] += qm[3] * t[c][2];
+        }
     }
-    const float wof = static_cast<float>(wo), ttf = static_cast<float>(tt);
-    float ss = wof * wof + ttf * ttf;
+    const float wof = static_cast<float>(wo);
+    float ss = wof * wof;
+    for (int c = 0; c < NT; ++c) {
+        const float ttf = static_cast<float>(tt[c]);
+        ss += ttf * ttf;
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
     const float s = sqrtf(ss / NFP) + 1e-30f;
     const float is = 1.0f / s;
     if (lane < NFP) {
         sx[lane] = wof * is;
-        sx[NFP + lane] = ttf * is;
+        for (int c = 0; c < NT; ++c) sx[(1 + c) * NFP + lane] = static_cast<float>(tt[c]) * is;
     }
     if (lane == 0) {
-        const float ic0 = static_cast<float>(T(1) / a.ph.c0);
-        sx[2 * NFP] = static_cast<float>(vn0) * ic0;
-        sx[2 * NFP + 1] = static_cast<float>(vt0) * ic0;
+        const float ic0 = static_cast<float>(T(1) / c0);
+        sx[DIM * NFP] = static_cast<float>(vn0) * ic0;
+        sx[DIM * NFP + 1] = static_cast<float>(vt0) * ic0;
     }
     waveFence();
     mlpForwardWave<NIN, NOUT>(net, sx, sa1, sa2, sy, lane);
     if (lane < NFP) {
-        const T c0 = a.ph.c0;
         const T what = static_cast<T>(s * sy[lane]);
         const T lam = vn0 - c0;
         const T wip = (T(2) * lam * what - (lam + c0) * wi) / (lam - c0);
-        const T tp = (vn0 >= T(0)) ? tt : -(vn0 + c0) * tt / (vn0 - c0);
+        T tp[NT];
+        for (int c = 0; c < NT; ++c) tp[c] = (vn0 >= T(0)) ? tt[c] : -(vn0 + c0) * tt[c] / (vn0 - c0);
         const T Pp = T(0.5) * (wo + wip), Np_ = T(0.5) * (wo - wip);
         T qp[NF];
         qp[0] = a.ph.rho0 * c0 * Pp;
-        qp[1] = Np_ * n0 + tp * t0;
-        qp[2] = Np_ * n1 + tp * t1;
-        qp[3] = qm[3];
+        for (int x = 0; x < DIM; ++x) {
+            T v = Np_ * n[x];
+            for (int c = 0; c < NT; ++c) v += tp[c] * t[c][x];
+            qp[1 + x] = v;
+        }
+        if constexpr (DIM == 2) qp[3] = qm[3];
         if constexpr (NF == 5) qp[4] = qm[4] + (qp[0] - qm[0]) * a.ph.ic2;
         T* g = a.ghost + (idx * NFP + lane) * NF;
 #pragma unroll

@@ -351,7 +351,7 @@ void launchPost(dg_context* c) {
                        p, sourceArgs(c), static_cast<T*>(c->u), static_cast<T*>(COMP ? c->ulo : nullptr));
 }
 
-template <int P, int NF, typename T>
+template <int DIM, int P, int NF, typename T>
 void launchClosure(dg_context* c, const void* qin) {
     dgc::ClosureArgs<T> a{};
     a.qin = static_cast<const T*>(qin);
@@ -363,7 +363,8 @@ void launchClosure(dg_context* c, const void* qin) {
     a.nfaces = c->nLearned;
     a.ph = makePhysics<T>(c->phys);
     const int64_t nb = (c->nLearned + dgc::kClosureFacesPerBlock - 1) / dgc::kClosureFacesPerBlock;
-    hipLaunchKernelGGL((dgc::closureKernel<P, NF, T>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL((dgc::closureKernel<DIM, P, NF, T>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, c->stream,
+                       a);
 }
 
 template <int DIM, int P, int NF, typename T>
@@ -379,10 +380,10 @@ template <int DIM, int P, int NF>
 void pickAux(dg_context* c) {
     if (c->precision == DG_FP64) {
         c->traceFn = &launchTraces<DIM, P, NF, double>;
-        if constexpr (DIM == 2) c->closureFn = &launchClosure<P, NF, double>;
+        if constexpr (DIM == 2 || P <= 3) c->closureFn = &launchClosure<DIM, P, NF, double>;
     } else {
         c->traceFn = &launchTraces<DIM, P, NF, float>;
-        if constexpr (DIM == 2) c->closureFn = &launchClosure<P, NF, float>;
+        if constexpr (DIM == 2 || P <= 3) c->closureFn = &launchClosure<DIM, P, NF, float>;
     }
 }
 
@@ -763,9 +764,10 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         c->Nfp = R.Nfp;
         c->Nfc = R.Nfaces;
         if (c->K > INT32_MAX / 2) throw std::runtime_error("too many elements for 32-bit neighbour ids");
-        if (!c->mesh.learnedFaces.empty() && m->dim != 2) {
+        if (!c->mesh.learnedFaces.empty() && m->dim == 3 && m->order > 3) {
             delete c;
-            g_createError = "dg_create: the learned closure (DG_BC_LEARNED) is built for 2D meshes";
+            g_createError = "dg_create: the learned closure (DG_BC_LEARNED) is built for 2D meshes and tetrahedra "
+                            "of order <= 3";
             return DG_ERR_UNSUPPORTED;
         }
         if (!pickKernels(c)) {
@@ -841,7 +843,7 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         if (c->nLearned > 0) {
             c->dLearned = uploadVec(c, c->mesh.learnedFaces);
             c->ghost = alloc(static_cast<size_t>(c->nLearned) * c->Nfp * c->nf);
-            c->closureParams = dgc::numParams(2 * c->Nfp + 2, c->Nfp);
+            c->closureParams = dgc::numParams(c->dim * c->Nfp + 2, c->Nfp);
             c->dClosureW = devAlloc<uint16_t>(c, c->closureParams);
             HIPCHK(hipMemset(c->dClosureW, 0, c->closureParams * sizeof(uint16_t)));
         }

@@ -123,7 +123,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "dg_closure_num_params": (C.c_int64, [C.c_int32, C.c_int32]),
         "dg_closure_last_error": (C.c_char_p, []),
         "dg_closure_to_bf16": (None, [P(C.c_float), C.c_int64, P(C.c_uint16)]),
-        "dg_closure_features": (C.c_int32, [C.c_int32, C.c_int32, C.c_int64, P(C.c_double), P(C.c_double),
+        "dg_closure_features": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int64, P(C.c_double), P(C.c_double),
                                             P(Physics), P(C.c_float), P(C.c_float), P(C.c_float)]),
         "dg_closure_forward": (C.c_int32, [C.c_int32, C.c_int32, P(C.c_uint16), P(C.c_float), C.c_int64,
                                            P(C.c_float), C.c_int32]),
@@ -272,14 +272,21 @@ def mesh_precompute(node_coords, element_vertices, order, face_bc=None) -> dict:
 
 # ------------------------------------------------------------ learned closure network ----
 
-def closure_sizes(nfp: int) -> tuple[int, int]:
-    """(n_in, n_out) of the closure network for faces with nfp nodes."""
-    return 2 * nfp + 2, nfp
+def closure_sizes(nfp: int, dim: int = 2) -> tuple[int, int]:
+    """(n_in, n_out) of the closure network for dim-dimensional faces with nfp nodes."""
+    return dim * nfp + 2, nfp
 
 
-def closure_num_params(nfp: int) -> int:
-    nin, nout = closure_sizes(nfp)
+def closure_num_params(nfp: int, dim: int = 2) -> int:
+    nin, nout = closure_sizes(nfp, dim)
     return int(load_library().dg_closure_num_params(nin, nout))
+
+
+def _closure_nout(nparams: int, nin: int) -> int:
+    r = nparams - (64 * nin + 64 + 64 * 64 + 64)
+    if r <= 0 or r % 65:
+        raise ValueError(f"{nparams} parameters do not fit a network with {nin} inputs")
+    return r // 65
 
 
 def to_bf16(x) -> np.ndarray:
@@ -295,18 +302,18 @@ def _closure_check(st):
         raise DGError(st, load_library().dg_closure_last_error().decode())
 
 
-def closure_features(traces, normals, v0, rho0, c0, with_targets=True):
-    """Features X [n][2nfp+2], targets Y [n][nfp] (or None) and scales s [n] of face traces
-    [n][2][nfp][nf] (interior, neighbour) with outward normals [n][3]."""
+def closure_features(traces, normals, v0, rho0, c0, with_targets=True, dim=2):
+    """Features X [n][dim nfp + 2], targets Y [n][nfp] (or None) and scales s [n] of face traces
+    [n][2][nfp][nf] (interior, neighbour) of dim-dimensional faces with outward normals [n][3]."""
     T = np.ascontiguousarray(traces, dtype=np.float64)
     Nn = np.ascontiguousarray(normals, dtype=np.float64)
     n, _, nfp, nf_ = T.shape
-    nin, nout = closure_sizes(nfp)
+    nin, nout = closure_sizes(nfp, dim)
     X = np.zeros((n, nin), dtype=np.float32)
     Y = np.zeros((n, nout), dtype=np.float32) if with_targets else None
     S = np.zeros(n, dtype=np.float32)
     ph = Physics((C.c_double * 3)(*v0), rho0, c0, DG_FIELDS_LEE5 if nf_ == 5 else DG_FIELDS_REF4)
-    st = load_library().dg_closure_features(nfp, nf_, n, _ptr(T, C.c_double), _ptr(Nn, C.c_double), C.byref(ph),
+    st = load_library().dg_closure_features(dim, nfp, nf_, n, _ptr(T, C.c_double), _ptr(Nn, C.c_double), C.byref(ph),
                                             _ptr(X, C.c_float), _ptr(Y, C.c_float) if Y is not None else None,
                                             _ptr(S, C.c_float))
     if st != DG_OK:
@@ -317,8 +324,8 @@ def closure_features(traces, normals, v0, rho0, c0, with_targets=True):
 def closure_forward(params_bf16, X, device=0) -> np.ndarray:
     X = np.ascontiguousarray(X, dtype=np.float32)
     n, nin = X.shape
-    nout = (nin - 2) // 2
     W = np.ascontiguousarray(params_bf16, dtype=np.uint16)
+    nout = _closure_nout(W.size, nin)
     Y = np.zeros((n, nout), dtype=np.float32)
     _closure_check(load_library().dg_closure_forward(nin, nout, _ptr(W, C.c_uint16), _ptr(X, C.c_float), n,
                                                      _ptr(Y, C.c_float), device))
@@ -339,10 +346,10 @@ def closure_loss_grad(params_bf16, X, Y, device=0):
     return loss.value, g
 
 
-def closure_init(nfp: int, seed: int = 0, out_scale: float = 0.0) -> np.ndarray:
+def closure_init(nfp: int, seed: int = 0, out_scale: float = 0.0, dim: int = 2) -> np.ndarray:
     """FP32 master parameters: Glorot-uniform hidden layers, output layer scaled by
     out_scale (0: the network starts as the zero map, i.e. the characteristic condition)."""
-    nin, nout = closure_sizes(nfp)
+    nin, nout = closure_sizes(nfp, dim)
     rng = np.random.default_rng(seed)
     H = 64
 

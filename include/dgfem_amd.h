@@ -149,15 +149,18 @@ DG_API double dg_last_run_device_ms(const dg_context* ctx);
 
 /* ---- learned absorbing-boundary closure (no counterpart in the reference; its
    characteristic absorbing BC, src/Mesh.cpp:619-666, stays the fallback) ----
-   Network: n_in = 2 Nfp + 2 -> 64 -> 64 -> n_out = Nfp, tanh, BF16 weights, FP32
+   Network: n_in = dim Nfp + 2 -> 64 -> 64 -> n_out = Nfp, tanh, BF16 weights, FP32
    accumulation; parameters in the order W1[64][n_in], b1[64], W2[64][64], b2[64],
-   W3[n_out][64], b3[n_out] (row-major, out x in). On a 2D face with outward normal n and
-   tangent t = (-n_y, n_x), per face node P = p/(rho0 c0), N = v.n, T = v.t and the
-   characteristics w_out = P + N, w_in = P - N: scale s = sqrt(mean(w_out^2 + T^2)) + 1e-30,
-   x = [w_out/s, T/s, v0.n/c0, v0.t/c0]; the network predicts the incoming characteristic
-   w_hat = s y, and the exterior state is chosen so that the face flux is the upwind flux
-   with incoming values w_hat (acoustic) and 0 (vorticity). The zero network is the exact
-   characteristic condition. */
+   W3[n_out][64], b3[n_out] (row-major, out x in). On a face with outward normal n and
+   tangents t_c (2D: t = (-n_y, n_x); 3D: t1 along the tangential mean flow
+   v0 - (v0.n) n when its norm exceeds 1e-6 c0, else along e_a - (e_a.n) n for the axis a least
+   aligned with n, t2 = n x t1), per face node P = p/(rho0 c0), N = v.n, T_c = v.t_c and the
+   characteristics w_out = P + N, w_in = P - N: scale s = sqrt(mean(w_out^2 + sum T_c^2))
+   + 1e-30, x = [w_out/s, T_1/s, (T_2/s,) v0.n/c0, vt0/c0] with vt0 = v0.t (2D) or
+   |v0 - (v0.n) n| (3D); the network predicts the incoming characteristic w_hat = s y, and
+   the exterior state is chosen so that the face flux is the upwind flux with incoming
+   values w_hat (acoustic) and 0 (vorticity). The zero network is the exact characteristic
+   condition. Shapes built: 2D Nfp = 2..6, 3D Nfp = 3, 6, 10. */
 
 /* Network of the context's DG_BC_LEARNED faces (default: all zero, i.e. exterior state =
    interior trace). params: BF16 bit patterns, n_params = dg_closure_num_params(...). */
@@ -177,9 +180,9 @@ DG_API const char* dg_closure_last_error(void);
 DG_API void dg_closure_to_bf16(const float* x, int64_t n, uint16_t* out);
 /* Features X [n][n_in], targets Y [n][n_out] (the neighbour trace's incoming characteristic
    divided by s; may be NULL) and scales S [n] (may be NULL) from traces [n][2][nfp][nf] and
-   normals [n][3] as returned by dg_face_traces. */
-DG_API dg_status dg_closure_features(int32_t nfp, int32_t nf, int64_t n, const double* traces, const double* normals,
-                                     const dg_physics* phys, float* X, float* Y, float* S);
+   normals [n][3] of dim-dimensional faces as returned by dg_face_traces. */
+DG_API dg_status dg_closure_features(int32_t dim, int32_t nfp, int32_t nf, int64_t n, const double* traces,
+                                     const double* normals, const dg_physics* phys, float* X, float* Y, float* S);
 /* Inference Y [n][n_out] of X [n][n_in] on `device`. */
 DG_API dg_status dg_closure_forward(int32_t n_in, int32_t n_out, const uint16_t* params, const float* X, int64_t n,
                                     float* Y, int32_t device);

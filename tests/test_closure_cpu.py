@@ -113,7 +113,9 @@ def test_exterior_state_gives_the_upwind_flux(vmag):
     qp = co.exterior_from_incoming(qi, nrm, w_hat, v0, rho0, c0)
     got = _lf_flux(qi, qp, nrm[:, :2], v0, rho0, c0)
     n, t = co.frame(nrm)
+    t = t[:, 0]
     wo, _, T = co.characteristics(qi, nrm, rho0, c0)
+    T = T[:, 0]
     vn0 = (n @ v0[:2])[:, None]
     FP = 0.5 * ((vn0 + c0) * wo + (vn0 - c0) * w_hat)
     FN = 0.5 * ((vn0 + c0) * wo - (vn0 - c0) * w_hat)
@@ -155,9 +157,111 @@ def test_learned_faces_numbering_in_precompute():
     assert order == list(range(16))  # numbered in (element, local face) order
 
 
-def test_learned_closure_rejected_in_3d():
-    m = dg.box_mesh(3, 1, 1, seed=1)
+def test_learned_closure_rejected_on_high_order_tets():
+    m = dg.box_mesh(3, 1, 4, seed=1)
     bc = np.full((m.K, 4), dg.DG_BC_LEARNED, dtype=np.int8)
     with pytest.raises(dg.DGError) as e:
-        dg.Solver(m.node_coords, m.element_vertices, 1, bc)
-    assert e.value.code == dg.DG_ERR_UNSUPPORTED and "2D" in str(e.value)
+        dg.Solver(m.node_coords, m.element_vertices, 4, bc)
+    assert e.value.code == dg.DG_ERR_UNSUPPORTED and "order <= 3" in str(e.value)
+
+
+def test_learned_faces_numbered_on_tets():
+    m = dg.box_mesh(3, 2, 2, seed=3)
+    bc = np.where(m.face_side >= 0, dg.DG_BC_LEARNED, 0).astype(np.int8)
+    nbr = dg.mesh_precompute(m.node_coords, m.element_vertices, 2, bc)["nbr"]
+    flat = nbr.ravel()
+    assert [-3 - v for v in flat if v <= -3] == list(range(6 * 2 * 2 * 2))  # 6 sides x 4 squares x 2 triangles
+    assert (flat >= -2).sum() + (flat <= -3).sum() == flat.size and not np.any((flat == -1) | (flat == -2))
+
+
+# ---- 3D faces ----
+
+def _random_faces3d(m, nfp, seed, nf=4):
+    rng = np.random.default_rng(seed)
+    n = rng.normal(0, 1, (m, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    qi = rng.normal(0, 1, (m, nfp, nf))
+    qe = qi + rng.normal(0, 0.1, (m, nfp, nf))
+    return qi, qe, n
+
+
+def _lf_flux3d(q_m, q_p, n, v0, rho0, c0):
+    """Lax-Friedrichs face flux in physical variables, 3D (the solver's interior flux)."""
+    def F(q):
+        vn0 = (n @ v0)[:, None]
+        vn = np.einsum("mjc,mc->mj", q[..., 1:4], n)
+        out = np.zeros_like(q)
+        out[..., 0] = vn0 * q[..., 0] + rho0 * c0 * c0 * vn
+        for r in range(3):
+            out[..., 1 + r] = vn0 * q[..., 1 + r] + n[:, r:r + 1] * q[..., 0] / rho0
+        return out
+    return 0.5 * (F(q_m) + F(q_p)) + 0.5 * c0 * (q_m - q_p)
+
+
+@pytest.mark.parametrize("vmag", [0.0, 60.0, 120.0])
+def test_exterior_state_gives_the_upwind_flux_3d(vmag):
+    """3D: the exterior state turns the Lax-Friedrichs flux into the upwind flux with the
+    prescribed incoming characteristic, outgoing (vn0 + c0) w_out-, and the tangential velocity
+    (both components) carried out on outflow and zero on inflow. The expected flux is built
+    from the eigenvectors of A.n written independently of the oracle's frame."""
+    rho0, c0 = 1.2, 340.0
+    rng = np.random.default_rng(int(vmag) + 7)
+    qi, _, nrm = _random_faces3d(40, 10, 19)
+    d = rng.normal(0, 1, 3)
+    v0 = vmag * d / np.linalg.norm(d)
+    w_hat = rng.normal(0, 1, (40, 10))
+    qp = co.exterior_from_incoming(qi, nrm, w_hat, v0, rho0, c0, dim=3)
+    got = _lf_flux3d(qi, qp, nrm, v0, rho0, c0)
+    vn0 = (nrm @ v0)[:, None]
+    v = qi[..., 1:4]
+    vn = np.einsum("mjc,mc->mj", v, nrm)
+    wo = qi[..., 0] / (rho0 * c0) + vn
+    vt = v - vn[..., None] * nrm[:, None, :]                 # tangential velocity, frame-free
+    FP = 0.5 * ((vn0 + c0) * wo + (vn0 - c0) * w_hat)
+    FN = 0.5 * ((vn0 + c0) * wo - (vn0 - c0) * w_hat)
+    FT = np.where((vn0 >= 0)[..., None], vn0[..., None] * vt, 0.0)
+    want = np.zeros_like(got)
+    want[..., 0] = rho0 * c0 * FP
+    want[..., 1:4] = FN[..., None] * nrm[:, None, :] + FT
+    np.testing.assert_allclose(got, want, atol=1e-9 * np.abs(want).max())
+
+
+def test_features_rotation_invariance_3d():
+    """Rotating traces, normals and mean flow together leaves the 3D features unchanged (the
+    tangent frame follows the tangential mean flow); the scale is homogeneous of degree one."""
+    rho0, c0 = 1.225, 340.0
+    qi, _, nrm = _random_faces3d(30, 10, 23)
+    v0 = np.array([40.0, -25.0, 60.0])
+    X, s = co.features(qi, nrm, v0, rho0, c0, dim=3)
+    A = np.random.default_rng(5).normal(0, 1, (3, 3))
+    R, _ = np.linalg.qr(A)
+    qr = qi.copy()
+    qr[..., 1:4] = qi[..., 1:4] @ R.T
+    Xr, sr = co.features(qr, nrm @ R.T, R @ v0, rho0, c0, dim=3)
+    np.testing.assert_allclose(Xr, X, atol=1e-12)
+    np.testing.assert_allclose(sr, s, rtol=1e-13)
+    X2, s2 = co.features(2.5 * qi, nrm, v0, rho0, c0, dim=3)
+    np.testing.assert_allclose(X2, X, atol=1e-12)
+    np.testing.assert_allclose(s2, 2.5 * s, rtol=1e-13)
+    # no tangential mean flow: the frame follows the coordinate axes; the features still see
+    # the same tangential speed |v_t| per node
+    Xa, sa = co.features(qi, nrm, (nrm[0] * 50.0), rho0, c0, dim=3)
+    T2 = Xa[:, 10:20] ** 2 + Xa[:, 20:30] ** 2
+    vt = qi[..., 1:4] - np.einsum("mjc,mc->mj", qi[..., 1:4], nrm)[..., None] * nrm[:, None, :]
+    np.testing.assert_allclose(T2 * sa[:, None] ** 2, (vt ** 2).sum(-1), rtol=1e-10)
+
+
+def test_library_features_match_oracle_3d():
+    rho0, c0 = 1.225, 340.0
+    for v0, nfp in (((50.0, 10.0, -30.0), 10), ((0.0, 0.0, 0.0), 6), ((20.0, 0.0, 0.0), 3)):
+        qi, qe, nrm = _random_faces3d(200, nfp, 31 + nfp)
+        nrm[:5] = np.eye(3)[[0, 1, 2, 0, 1]] * np.array([1, -1, 1, -1, 1])[:, None]  # axis-aligned faces
+        traces = np.stack([qi, qe], axis=1)
+        X, Y, S = dg.closure_features(traces, nrm, v0, rho0, c0, dim=3)
+        Xo, s = co.features(qi, nrm, v0, rho0, c0, dim=3)
+        Yo = co.targets(qe, nrm, rho0, c0, s, v0, dim=3)
+        assert X.shape == (200, 3 * nfp + 2)
+        np.testing.assert_allclose(S, s, rtol=2e-6)
+        np.testing.assert_allclose(X, Xo, rtol=2e-6, atol=2e-6)
+        np.testing.assert_allclose(Y, Yo, rtol=2e-5, atol=2e-6)
+    assert dg.closure_num_params(10, dim=3) == 64 * 32 + 64 + 64 * 64 + 64 + 10 * 64 + 10

@@ -18,29 +18,29 @@ from cases import rel_l2, smooth_state
 pytestmark = pytest.mark.gpu
 
 
-def _params(nfp, seed, scale=0.3):
+def _params(nfp, seed, scale=0.3, dim=2):
     rng = np.random.default_rng(seed)
-    m = rng.normal(0, scale, dg.closure_num_params(nfp)).astype(np.float32)
+    m = rng.normal(0, scale, dg.closure_num_params(nfp, dim)).astype(np.float32)
     bits = dg.to_bf16(m)
     return bits, co.bf16_to_f64(bits)
 
 
-@pytest.mark.parametrize("nfp", [2, 4, 5, 6])
-def test_forward_matches_oracle(nfp):
-    print(f"case closure forward nfp={nfp}", flush=True)
-    nin, nout = co.sizes(nfp)
-    bits, p64 = _params(nfp, nfp)
+@pytest.mark.parametrize("nfp,dim", [(2, 2), (4, 2), (5, 2), (6, 2), (3, 3), (6, 3), (10, 3)])
+def test_forward_matches_oracle(nfp, dim):
+    print(f"case closure forward nfp={nfp} dim={dim}", flush=True)
+    nin, nout = co.sizes(nfp, dim)
+    bits, p64 = _params(nfp, nfp, dim=dim)
     X = np.random.default_rng(1).normal(0, 1, (3000, nin)).astype(np.float32)
     y = dg.closure_forward(bits, X)
     yo, _ = co.forward(p64, X.astype(np.float64))
     assert np.abs(y - yo).max() <= 2e-5 * max(1.0, np.abs(yo).max())
 
 
-@pytest.mark.parametrize("nfp,n", [(3, 1), (3, 777), (5, 5000)])
-def test_loss_and_gradient_match_oracle(nfp, n):
-    print(f"case closure grad nfp={nfp} n={n}", flush=True)
-    nin, nout = co.sizes(nfp)
-    bits, p64 = _params(nfp, 10 + nfp)
+@pytest.mark.parametrize("nfp,n,dim", [(3, 1, 2), (3, 777, 2), (5, 5000, 2), (10, 3001, 3)])
+def test_loss_and_gradient_match_oracle(nfp, n, dim):
+    print(f"case closure grad nfp={nfp} n={n} dim={dim}", flush=True)
+    nin, nout = co.sizes(nfp, dim)
+    bits, p64 = _params(nfp, 10 + nfp, dim=dim)
     rng = np.random.default_rng(2)
     X = rng.normal(0, 1, (n, nin)).astype(np.float32)
     Y = rng.normal(0, 0.5, (n, nout)).astype(np.float32)
@@ -172,3 +172,47 @@ def test_learned_closure_beats_characteristic_bc():
     assert np.all(np.isfinite(learned)) and np.all(learned < 1.25 * ref_bc), (learned, ref_bc)
     assert learned.mean() < ref_bc.mean(), (learned, ref_bc)
     assert learned.mean() <= 1.05 * exact.mean(), (learned, exact)
+
+
+# ---- tetrahedra ----
+
+def _learned_case3d(n, p, seed=6):
+    m = dg.box_mesh(3, n, p, jitter=0.15, seed=seed)
+    bc = np.where(m.face_side >= 0, dg.DG_BC_LEARNED, dg.DG_BC_ABSORBING).astype(np.int8)
+    return m, bc
+
+
+@pytest.mark.parametrize("p,precision,v0", [(2, dg.DG_FP64, (40.0, -25.0, 15.0)), (3, dg.DG_FP64, (0.0, 0.0, 0.0)),
+                                            (3, dg.DG_MIXED, (30.0, 0.0, 0.0))])
+def test_rhs_with_learned_faces_matches_nodal_restatement_3d(p, precision, v0):
+    """Tetrahedra: the closure kernel's exterior states (3D face frame, two tangential
+    components) against the oracle's, through the whole right-hand side. v0 = 0 exercises the
+    axis-based tangents; (30, 0, 0) has faces with and without tangential mean flow."""
+    print(f"case closure rhs 3d p={p} precision={precision} v0={v0}", flush=True)
+    rho0, c0 = 1.225, 340.0
+    m, bc = _learned_case3d(3, p)
+    nfp = (p + 1) * (p + 2) // 2
+    bits, p64 = _params(nfp, 8 + p, scale=0.2, dim=3)
+    s = dg.Solver(m.node_coords, m.element_vertices, p, bc, v0=v0, rho0=rho0, c0=c0, precision=precision)
+    e, f = s.learned_faces()
+    assert e.size == 6 * 9 * 2
+    s.set_closure(bits)
+    u = smooth_state(m)
+    k = s.rhs(u, 1e-5)
+    pre = dg.mesh_precompute(m.node_coords, m.element_vertices, p, bc)
+    ops = dg.ref_operators(3, p)
+    kr = nodal_ref.nodal_rhs(u, pre, ops, 3, v0, rho0, c0, 1e-5,
+                             closure=lambda qm, nn: co.exterior(qm, nn, p64, v0, rho0, c0, dim=3))
+    k0 = nodal_ref.nodal_rhs(u, pre, ops, 3, v0, rho0, c0, 1e-5,
+                             closure=lambda qm, nn: co.exterior_from_incoming(qm, nn, 0.0, v0, rho0, c0, dim=3))
+    tol = 1e-6 if precision == dg.DG_FP64 else 2e-5
+    for fld in range(4):
+        assert rel_l2(k[fld], kr[fld]) < tol, (fld, rel_l2(k[fld], kr[fld]))
+    assert max(rel_l2(kr[fld], k0[fld]) for fld in range(4)) > 100 * tol
+    # the default (zero) network is the exact characteristic condition
+    s2 = dg.Solver(m.node_coords, m.element_vertices, p, bc, v0=v0, rho0=rho0, c0=c0, precision=precision)
+    k2 = s2.rhs(u, 1e-5)
+    for fld in range(4):
+        assert rel_l2(k2[fld], k0[fld]) < (1e-12 if precision == dg.DG_FP64 else 2e-6)
+    s.close()
+    s2.close()
