@@ -2,9 +2,12 @@
 training with the library's GPU kernels, and evaluation against the characteristic
 absorbing BC of the reference (src/Mesh.cpp:619-666, :954-969) on the truncated domain.
 
-The truncated mesh is the part of the enlarged mesh inside [-1,1]^2, element for element
-(vertices on the lines x = +-1, y = +-1 are kept on them), so that a truncated run differs
-from the enlarged one only by what happens on the truncation faces. The network, its
+The truncated mesh is the part of the enlarged mesh inside [-1,1]^dim, element for element
+(vertices on the lines / planes x = +-1, y = +-1 (, z = +-1) are kept on them), so that a
+truncated run differs from the enlarged one only by what happens on the truncation faces.
+2D: triangles, monopoles or Gaussian pulses, observers on a circle. 3D: Kuhn tetrahedra,
+dipole or quadrupole sources (expanded into antiphase monopoles as the reference's config
+parser does, src/configParser.cpp:87-108), observers at Fibonacci points of a sphere. The network, its
 training kernels and the in-solver inference live in libdgfem_amd.so (dg_closure_*,
 dg_set_closure); this module drives them.
 """
@@ -20,19 +23,21 @@ import dgfem_amd as dg
 
 @dataclass
 class ClosureCase:
-    n_trunc: int = 100          # cells per side of [-1,1]^2 (h = 2 / n_trunc)
-    factor: int = 3             # enlarged domain [-factor, factor]^2
+    n_trunc: int = 100          # cells per side of [-1,1]^dim (h = 2 / n_trunc)
+    factor: int = 3             # enlarged domain [-factor, factor]^dim
     order: int = 4
+    dim: int = 2
     jitter: float = 0.2
     seed: int = 4
     c0: float = 340.0
     rho0: float = 1.225
-    source_size: float = 0.05   # monopole node radius
-    n_observers: int = 8        # on the circle r = observer_radius
+    source_size: float = 0.05   # monopole node radius (3D: dipole / quadrupole pole offset)
+    n_observers: int = 8        # on the circle / sphere r = observer_radius
     observer_radius: float = 0.8
     observer_size: float = 0.03
     cfl: float = 1.0            # dt = cfl * r_min / ((c0 + |v0|max) (2p + 1))
     vmax: float = 0.3           # |v0| <= vmax * c0
+    precision: int = dg.DG_FP64  # precision of the truncated-domain runs
     travel: float = 3.0         # simulated time = travel / c0 (source -> boundary -> observers)
     steps: int = 0              # 0: from travel and dt
 
@@ -50,26 +55,44 @@ class Meshes:
 
 
 def _snap(mesh: dg.BoxMesh, lo: float, h: float, lines=(-1.0, 1.0)) -> np.ndarray:
-    """Node coordinates with every vertex whose grid line is x = +-1 or y = +-1 put back on
-    that line; the other nodes are rebuilt from the element's (moved) vertices."""
+    """Node coordinates with every vertex whose grid line (plane) is x = +-1, y = +-1 (or
+    z = +-1) put back on it; the other nodes are rebuilt from the element's (moved) vertices."""
     X = mesh.node_coords
     K, Np, _ = X.shape
-    V = X[:, :3, :2]
+    d = mesh.dim
+    V = X[:, :d + 1, :d]
     # barycentric coordinates of the element nodes (same local pattern for every element)
-    A = np.stack([V[0, 1] - V[0, 0], V[0, 2] - V[0, 0]], axis=1)
-    xi = np.linalg.solve(A, (X[0, :, :2] - V[0, 0]).T).T
-    lam = np.stack([1 - xi[:, 0] - xi[:, 1], xi[:, 0], xi[:, 1]], axis=1)
+    A = np.stack([V[0, k + 1] - V[0, 0] for k in range(d)], axis=1)
+    xi = np.linalg.solve(A, (X[0, :, :d] - V[0, 0]).T).T
+    lam = np.concatenate([1 - xi.sum(axis=1, keepdims=True), xi], axis=1)
     rebuilt = np.einsum("ak,ekc->eac", lam, V)
-    assert np.abs(rebuilt - X[:, :, :2]).max() < 1e-10, "generator node pattern is not affine-uniform"
+    assert np.abs(rebuilt - X[:, :, :d]).max() < 1e-10, "generator node pattern is not affine-uniform"
     Vs = V.copy()
-    for c in range(2):
+    for c in range(d):
         g = np.rint((Vs[..., c] - lo) / h)
         for ln in lines:
             on = np.abs(g * h + lo - ln) < 1e-9
             Vs[..., c][on] = ln
     out = X.copy()
-    out[:, :, :2] = np.einsum("ak,ekc->eac", lam, Vs)
+    out[:, :, :d] = np.einsum("ak,ekc->eac", lam, Vs)
     return out
+
+
+def _inradius(V):
+    """Inradius of triangles [K][3][2] or tetrahedra [K][4][3]."""
+    if V.shape[1] == 3:
+        a = np.linalg.norm(V[:, 1] - V[:, 2], axis=1)
+        b = np.linalg.norm(V[:, 0] - V[:, 2], axis=1)
+        c = np.linalg.norm(V[:, 0] - V[:, 1], axis=1)
+        area = 0.5 * np.abs(_cross2(V[:, 1] - V[:, 0], V[:, 2] - V[:, 0]))
+        return 2 * area / (a + b + c)
+    vol = np.abs(np.einsum("ij,ij->i", V[:, 1] - V[:, 0], np.cross(V[:, 2] - V[:, 0], V[:, 3] - V[:, 0]))) / 6
+
+    def tri(p, q, r):
+        return 0.5 * np.linalg.norm(np.cross(q - p, r - p), axis=1)
+    S = tri(V[:, 0], V[:, 1], V[:, 2]) + tri(V[:, 0], V[:, 1], V[:, 3]) + tri(V[:, 0], V[:, 2], V[:, 3]) + \
+        tri(V[:, 1], V[:, 2], V[:, 3])
+    return 3 * vol / S
 
 
 def _cross2(a, b):
@@ -78,36 +101,34 @@ def _cross2(a, b):
 
 def build_meshes(case: ClosureCase) -> Meshes:
     F = case.factor
+    d = case.dim
     n = case.n_trunc * F
     h = 2.0 / case.n_trunc
-    m = dg.box_mesh(2, n, case.order, jitter=case.jitter, seed=case.seed, lo=-float(F), hi=float(F))
+    m = dg.box_mesh(d, n, case.order, jitter=case.jitter, seed=case.seed, lo=-float(F), hi=float(F))
     coords = _snap(m, -float(F), h)
-    cen = coords[:, :3, :2].mean(axis=1)
-    inside = np.where((np.abs(cen[:, 0]) < 1) & (np.abs(cen[:, 1]) < 1))[0]
+    cen = coords[:, :d + 1, :d].mean(axis=1)
+    inside = np.where(np.all(np.abs(cen) < 1, axis=1))[0]
     sub_coords = np.ascontiguousarray(coords[inside])
     sub_verts = np.ascontiguousarray(m.element_vertices[inside])
     pre = dg.mesh_precompute(sub_coords, sub_verts, case.order)
     e, f = np.nonzero(pre["nbr"] < 0)
-    # inradius of the sub-mesh elements (CFL)
-    V = sub_coords[:, :3, :2]
-    a = np.linalg.norm(V[:, 1] - V[:, 2], axis=1)
-    b = np.linalg.norm(V[:, 0] - V[:, 2], axis=1)
-    c = np.linalg.norm(V[:, 0] - V[:, 1], axis=1)
-    area = 0.5 * np.abs(_cross2(V[:, 1] - V[:, 0], V[:, 2] - V[:, 0]))
-    r_in = 2 * area / (a + b + c)
-    Vp = coords[:, :3, :2]
-    ap = np.linalg.norm(Vp[:, 1] - Vp[:, 2], axis=1) + np.linalg.norm(Vp[:, 0] - Vp[:, 2], axis=1) + \
-        np.linalg.norm(Vp[:, 0] - Vp[:, 1], axis=1)
-    areap = 0.5 * np.abs(_cross2(Vp[:, 1] - Vp[:, 0], Vp[:, 2] - Vp[:, 0]))
-    r_min = float(min(r_in.min(), (2 * areap / ap).min()))
+    r_min = float(min(_inradius(sub_coords[:, :d + 1, :d]).min(), _inradius(coords[:, :d + 1, :d]).min()))
     return Meshes(coords, m.element_vertices, sub_coords, sub_verts, inside, (e.astype(np.int64), f.astype(np.int32)),
                   (inside[e].astype(np.int64), f.astype(np.int32)), r_min)
 
 
 def observers(case: ClosureCase):
-    ang = 2 * np.pi * (np.arange(case.n_observers) + 0.5) / case.n_observers
     r = case.observer_radius
-    return [(r * np.cos(a), r * np.sin(a), 0.0, case.observer_size) for a in ang]
+    if case.dim == 2:
+        ang = 2 * np.pi * (np.arange(case.n_observers) + 0.5) / case.n_observers
+        return [(r * np.cos(a), r * np.sin(a), 0.0, case.observer_size) for a in ang]
+    # Fibonacci points of the sphere
+    k = np.arange(case.n_observers) + 0.5
+    z = 1 - 2 * k / case.n_observers
+    phi = np.pi * (3 - np.sqrt(5)) * k
+    rho = np.sqrt(1 - z * z)
+    return [(r * rho[i] * np.cos(phi[i]), r * rho[i] * np.sin(phi[i]), r * z[i], case.observer_size)
+            for i in range(case.n_observers)]
 
 
 def time_step(case: ClosureCase, meshes: Meshes) -> tuple[float, int]:
@@ -120,6 +141,8 @@ def scenario(case: ClosureCase, seed: int) -> dict:
     """A monopole in the disk r < 0.5 at 20-200 Hz, or a Gaussian pulse of width 0.05-0.2,
     with a mean flow of |v0| <= vmax c0 in a random direction."""
     rng = np.random.default_rng(seed)
+    if case.dim == 3:
+        return _scenario3d(case, rng)
     r, a = 0.5 * np.sqrt(rng.uniform()), rng.uniform(0, 2 * np.pi)
     va, vm = rng.uniform(0, 2 * np.pi), case.vmax * case.c0 * np.sqrt(rng.uniform())
     s = dict(x=r * np.cos(a), y=r * np.sin(a), v0=(vm * np.cos(va), vm * np.sin(va), 0.0))
@@ -130,11 +153,38 @@ def scenario(case: ClosureCase, seed: int) -> dict:
     return s
 
 
+def _scenario3d(case: ClosureCase, rng) -> dict:
+    """3D: a dipole or a quadrupole at 50-300 Hz centred in the ball r < 0.5, mean flow
+    |v0| <= vmax c0 in a random direction."""
+    c = rng.normal(0, 1, 3)
+    c *= 0.5 * rng.uniform() ** (1 / 3) / np.linalg.norm(c)
+    d = rng.normal(0, 1, 3)
+    vm = case.vmax * case.c0 * rng.uniform() ** (1 / 3)
+    return dict(kind="dipole" if rng.uniform() < 0.5 else "quadrupole", x=c[0], y=c[1], z=c[2],
+                freq=rng.uniform(50, 300), amp=1.0, v0=tuple(vm * d / np.linalg.norm(d)))
+
+
+def poles(scen, size):
+    """Monopoles (x, y, z, radius, phase) of a dipole / quadrupole of pole offset `size`, as the
+    reference's parser expands them (src/configParser.cpp:87-108): a dipole is two monopoles of
+    radius size/2 at x -+ size in antiphase; a quadrupole adds the pair at y -+ size in antiphase
+    with the first."""
+    x, y, z = scen["x"], scen["y"], scen["z"]
+    out = [(x - size, y, z, size / 2, 0.0), (x + size, y, z, size / 2, np.pi)]
+    if scen["kind"] == "quadrupole":
+        out = [(x - size, y, z, size / 2, 0.0), (x + size, y, z, size / 2, 0.0),
+               (x, y - size, z, size / 2, np.pi), (x, y + size, z, size / 2, np.pi)]
+    return out
+
+
 def _solver(coords, verts, case, scen, face_bc=None, precision=dg.DG_FP64):
     s = dg.Solver(coords, verts, case.order, face_bc, v0=scen["v0"], rho0=case.rho0, c0=case.c0, precision=precision,
                   steps_per_graph=16)
     if scen["kind"] == "monopole":
         s.add_monopole(scen["x"], scen["y"], 0.0, case.source_size, scen["amp"], scen["freq"], 0.0, 1e9)
+    elif scen["kind"] in ("dipole", "quadrupole"):
+        for (px, py, pz, rad, ph) in poles(scen, case.source_size):
+            s.add_monopole(px, py, pz, rad, scen["amp"], scen["freq"], ph, 1e9)
     else:
         s.add_gaussian(scen["x"], scen["y"], 0.0, scen["width"] ** 2, scen["amp"])
     for o in observers(case):
@@ -158,7 +208,7 @@ def enlarged_run(case, meshes, scen, dt, steps, record_every=1, face_fraction=1.
             sel = np.arange(e.size) if face_fraction >= 1 else np.sort(
                 rng.choice(e.size, max(1, int(face_fraction * e.size)), replace=False))
             tr, nrm = s.face_traces(e[sel], f[sel])
-            X, Y, S = dg.closure_features(tr, nrm, scen["v0"], case.rho0, case.c0)
+            X, Y, S = dg.closure_features(tr, nrm, scen["v0"], case.rho0, case.c0, dim=case.dim)
             Xs.append(X)
             Ys.append(Y)
             Ss.append(S)
@@ -175,11 +225,12 @@ def enlarged_run(case, meshes, scen, dt, steps, record_every=1, face_fraction=1.
     return np.concatenate(obs), X[keep], Y
 
 
-def truncated_run(case, meshes, scen, dt, steps, mode, params=None, precision=dg.DG_FP64):
+def truncated_run(case, meshes, scen, dt, steps, mode, params=None, precision=None):
     """mode: 'characteristic' (the reference's absorbing BC) or 'learned' (params: BF16
     network; None = the zero network, i.e. the exact characteristic condition)."""
     K = meshes.sub_verts.shape[0]
-    bc = np.zeros((K, 3), dtype=np.int8)
+    precision = case.precision if precision is None else precision
+    bc = np.zeros((K, case.dim + 1), dtype=np.int8)
     if mode == "learned":
         e, f = meshes.faces_sub
         bc[e, f] = dg.DG_BC_LEARNED
@@ -209,10 +260,10 @@ class Report:
     eval: list = field(default_factory=list)
 
 
-def scale_output(master: np.ndarray, nfp: int, alpha: float) -> np.ndarray:
+def scale_output(master: np.ndarray, nfp: int, alpha: float, dim: int = 2) -> np.ndarray:
     """Network with its output layer (W3, b3) multiplied by alpha: the learned incoming
     characteristic is blended with the exact characteristic condition (alpha = 0)."""
-    nin, _ = dg.closure_sizes(nfp)
+    nin, _ = dg.closure_sizes(nfp, dim)
     off = 64 * nin + 64 + 64 * 64 + 64
     m = np.array(master, dtype=np.float32, copy=True)
     m[off:] *= alpha
@@ -240,8 +291,8 @@ def train_and_evaluate(case: ClosureCase, n_train: int, n_eval: int, n_val: int 
     X = np.concatenate(Xs)
     Y = np.concatenate(Ys)
     del Xs, Ys
-    nfp = case.order + 1
-    master = dg.closure_init(nfp, seed=case.seed)
+    nfp = case.order + 1 if case.dim == 2 else (case.order + 1) * (case.order + 2) // 2
+    master = dg.closure_init(nfp, seed=case.seed, dim=case.dim)
     t0 = time.perf_counter()
     master, _, hist = dg.closure_train(master, X, Y, epochs=epochs, batch=batch, lr=lr, seed=case.seed)
     t_train = time.perf_counter() - t0
@@ -257,14 +308,14 @@ def train_and_evaluate(case: ClosureCase, n_train: int, n_eval: int, n_val: int 
         val.append((scen, ref))
     best = None
     for a in alphas:
-        w = dg.to_bf16(scale_output(master, nfp, a))
+        w = dg.to_bf16(scale_output(master, nfp, a, case.dim))
         errs = [rel_l2(truncated_run(case, meshes, sc, dt, steps, "learned", w)[0], ref) for sc, ref in val]
         rep.validation[str(a)] = errs
         log(f"  validation alpha={a}: observer rel L2 {['%.3e' % e for e in errs]}")
         if best is None or np.mean(errs) < best[0]:
             best = (float(np.mean(errs)), a)
     rep.alpha = best[1]
-    params = dg.to_bf16(scale_output(master, nfp, rep.alpha))
+    params = dg.to_bf16(scale_output(master, nfp, rep.alpha, case.dim))
     rep.params = params
     rep.master = master
     for j in range(n_eval):
@@ -289,9 +340,13 @@ def main(argv=None):
     import json
     ap = argparse.ArgumentParser(description="Train the learned absorbing closure on enlarged-domain runs and "
                                              "compare it with the characteristic BC on the truncated domain.")
-    ap.add_argument("--n-trunc", type=int, default=100, help="cells per side of [-1,1]^2 (h = 2/n)")
+    ap.add_argument("--dim", type=int, default=2, choices=(2, 3))
+    ap.add_argument("--n-trunc", type=int, default=100, help="cells per side of [-1,1]^dim (h = 2/n)")
     ap.add_argument("--order", type=int, default=4)
-    ap.add_argument("--factor", type=int, default=3, help="enlarged domain [-factor, factor]^2")
+    ap.add_argument("--factor", type=int, default=3, help="enlarged domain [-factor, factor]^dim")
+    ap.add_argument("--source-size", type=float, default=0.05,
+                    help="monopole radius (2D) / dipole-quadrupole pole offset (3D)")
+    ap.add_argument("--mixed", action="store_true", help="truncated-domain runs in mixed precision")
     ap.add_argument("--train", type=int, default=32)
     ap.add_argument("--val", type=int, default=2)
     ap.add_argument("--eval", type=int, default=4)
@@ -303,7 +358,8 @@ def main(argv=None):
     ap.add_argument("--observer-size", type=float, default=0.03)
     ap.add_argument("--out", default="", help="write the report as JSON here")
     a = ap.parse_args(argv)
-    case = ClosureCase(n_trunc=a.n_trunc, order=a.order, factor=a.factor, observer_size=a.observer_size)
+    case = ClosureCase(n_trunc=a.n_trunc, order=a.order, factor=a.factor, observer_size=a.observer_size, dim=a.dim,
+                       source_size=a.source_size, precision=dg.DG_MIXED if a.mixed else dg.DG_FP64)
     rep = train_and_evaluate(case, a.train, a.eval, a.val, a.epochs, a.batch, a.lr, a.record_every, a.face_fraction,
                              log=lambda s: print(s, flush=True))
     out = dict(rep.__dict__)
