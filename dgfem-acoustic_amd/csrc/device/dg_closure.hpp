@@ -75,31 +75,55 @@ struct NetLds {
     static constexpr int kTotal = kB3 + NOUT;
 };
 
-// Block-cooperative load of BF16 parameters (canonical order) into the LDS image.
+// LDS offset of parameter x (canonical order) in the image.
 template <int NIN, int NOUT>
-__device__ inline void loadNet(const uint16_t* __restrict__ w, float* s, int tid, int nthreads) {
+__device__ inline int netOffset(int x) {
     using N = NetLds<NIN, NOUT>;
-    constexpr int P = numParams(NIN, NOUT);
-    for (int x = tid; x < P; x += nthreads) {
-        const float v = bf16ToFloat(w[x]);
-        int o;
-        if (x < kH * NIN) {
-            o = N::kW1 + (x / NIN) * N::S1 + x % NIN;
-        } else if (x < kH * NIN + kH) {
-            o = N::kB1 + (x - kH * NIN);
-        } else if (x < kH * NIN + kH + kH * kH) {
-            const int y = x - (kH * NIN + kH);
-            o = N::kW2 + (y / kH) * N::S2 + y % kH;
-        } else if (x < kH * NIN + 2 * kH + kH * kH) {
-            o = N::kB2 + (x - (kH * NIN + kH + kH * kH));
-        } else if (x < kH * NIN + 2 * kH + kH * kH + NOUT * kH) {
-            const int y = x - (kH * NIN + 2 * kH + kH * kH);
-            o = N::kW3 + (y / kH) * N::S2 + y % kH;
-        } else {
-            o = N::kB3 + (x - (kH * NIN + 2 * kH + kH * kH + NOUT * kH));
-        }
-        s[o] = v;
+    if (x < kH * NIN) return N::kW1 + (x / NIN) * N::S1 + x % NIN;
+    if (x < kH * NIN + kH) return N::kB1 + (x - kH * NIN);
+    if (x < kH * NIN + kH + kH * kH) {
+        const int y = x - (kH * NIN + kH);
+        return N::kW2 + (y / kH) * N::S2 + y % kH;
     }
+    if (x < kH * NIN + 2 * kH + kH * kH) return N::kB2 + (x - (kH * NIN + kH + kH * kH));
+    if (x < kH * NIN + 2 * kH + kH * kH + NOUT * kH) {
+        const int y = x - (kH * NIN + 2 * kH + kH * kH);
+        return N::kW3 + (y / kH) * N::S2 + y % kH;
+    }
+    return N::kB3 + (x - (kH * NIN + 2 * kH + kH * kH + NOUT * kH));
+}
+
+// Block-cooperative load of BF16 parameters (canonical order) into the LDS image by NT
+// threads: every 16-byte chunk of the thread is loaded (clamped addresses) before the first
+// LDS write, then the trailing P mod 8 values. The caller publishes the image with a barrier.
+template <int NIN, int NOUT, int NT>
+__device__ inline void loadNet(const uint16_t* __restrict__ w, float* s, int tid) {
+    constexpr int P = numParams(NIN, NOUT);
+    constexpr int NC = P / 8, NTAIL = P - 8 * NC;
+    constexpr int IT = (NC + NT - 1) / NT;
+    typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+    U4 r[IT];
+#pragma unroll
+    for (int c = 0; c < IT; ++c) {
+        const int x = tid + c * NT;
+        r[c] = reinterpret_cast<const U4*>(w)[x < NC ? x : NC - 1];
+    }
+    uint16_t rt = 0;
+    if constexpr (NTAIL > 0) rt = w[8 * NC + (tid < NTAIL ? tid : 0)];
+#pragma unroll
+    for (int c = 0; c < IT; ++c) {
+        const int x = tid + c * NT;
+        if (x < NC) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t word = r[c][k >> 1];
+                const uint16_t h = static_cast<uint16_t>((k & 1) ? (word >> 16) : (word & 0xffffu));
+                s[netOffset<NIN, NOUT>(8 * x + k)] = bf16ToFloat(h);
+            }
+        }
+    }
+    if constexpr (NTAIL > 0)
+        if (tid < NTAIL) s[netOffset<NIN, NOUT>(8 * NC + tid)] = bf16ToFloat(rt);
 }
 
 __device__ inline void waveFence() {
@@ -165,15 +189,15 @@ __global__ __launch_bounds__(256) void closureKernel(ClosureArgs<T> a) {
     __shared__ float net[N::kTotal];
     __shared__ float buf[kClosureFacesPerBlock][NIN + 2 * kH + NOUT + 2];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    loadNet<NIN, NOUT>(a.w, net, tid, 256);
-    __syncthreads();
     const int64_t idx = static_cast<int64_t>(blockIdx.x) * kClosureFacesPerBlock + wave;
+    const int64_t ef = a.faces[idx < a.nfaces ? idx : a.nfaces - 1];  // issued with the network loads
+    loadNet<NIN, NOUT, 256>(a.w, net, tid);
+    __syncthreads();
     if (idx >= a.nfaces) return;  // whole wave leaves; no block barrier follows
     float* sx = buf[wave];
     float* sa1 = sx + NIN;
     float* sa2 = sa1 + kH;
     float* sy = sa2 + kH;
-    const int64_t ef = a.faces[idx];
     const int64_t e = ef / NFC;
     const int f = static_cast<int>(ef - e * NFC);
     const T c0 = a.ph.c0;
@@ -326,7 +350,7 @@ __global__ __launch_bounds__(64) void forwardKernel(const uint16_t* __restrict__
     __shared__ float net[NetLds<NIN, NOUT>::kTotal];
     __shared__ float sx[NIN], sa1[kH], sa2[kH], sy[NOUT];
     const int lane = threadIdx.x;
-    loadNet<NIN, NOUT>(w, net, lane, 64);
+    loadNet<NIN, NOUT, 64>(w, net, lane);
     __syncthreads();
     for (int64_t s = blockIdx.x; s < n; s += gridDim.x) {
         if (lane < NIN) sx[lane] = X[s * NIN + lane];
@@ -351,7 +375,7 @@ __global__ __launch_bounds__(64) void gradKernel(const uint16_t* __restrict__ w,
     __shared__ float net[N::kTotal];
     __shared__ float sx[NIN], sa1[kH], sa2[kH], sy[NOUT], sd3[NOUT], sd2[kH];
     const int lane = threadIdx.x;
-    loadNet<NIN, NOUT>(w, net, lane, 64);
+    loadNet<NIN, NOUT, 64>(w, net, lane);
     __syncthreads();
     float g1[NIN], g2[kH], g3[NOUT];
     float gb1 = 0.f, gb2 = 0.f, gb3 = 0.f;
