@@ -216,3 +216,23 @@ def test_rhs_with_learned_faces_matches_nodal_restatement_3d(p, precision, v0):
         assert rel_l2(k2[fld], k0[fld]) < (1e-12 if precision == dg.DG_FP64 else 2e-6)
     s.close()
     s2.close()
+
+
+def test_graph_replay_equals_eager_with_learned_faces_3d():
+    print("case closure graph vs eager 3d mixed", flush=True)
+    v0, rho0, c0 = (20.0, 10.0, -15.0), 1.225, 340.0
+    m, bc = _learned_case3d(4, 3)
+    bits, _ = _params(10, 12, scale=0.2, dim=3)
+    out = []
+    for graph in (True, False):
+        s = dg.Solver(m.node_coords, m.element_vertices, 3, bc, v0=v0, rho0=rho0, c0=c0, use_graph=graph,
+                      steps_per_graph=8, precision=dg.DG_MIXED)
+        s.set_closure(bits)
+        s.add_gaussian(0.1, -0.2, 0.05, 0.05, 1.0)
+        s.add_observer(0.6, 0.0, 0.0, 0.2)
+        r = s.run_steps(40, 1e-6)
+        out.append((s.get_state(), r.observers))
+        s.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    assert np.isfinite(out[0][0]).all() and np.abs(out[0][0][0]).max() > 0
