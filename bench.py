@@ -34,6 +34,20 @@ HBM_PEAK_GBPS = 8000.0
 BYTES_PER_DOF = {"fp64": 212.0, "mixed": 115.0}
 
 
+def copy_bandwidth(device=0, nbytes=2 << 30, reps=10):
+    """On-box HBM reference: GB/s (read + write) of a streaming 16-byte copy kernel
+    (dgfem-acoustic_amd/libdgfem_probe.so), or None if it is not built."""
+    import ctypes as C
+    path = os.path.join(ROOT, "dgfem-acoustic_amd", "libdgfem_probe.so")
+    if not os.path.exists(path):
+        return None
+    lib = C.CDLL(path)
+    lib.dgprobe_copy_gbps.restype = C.c_double
+    lib.dgprobe_copy_gbps.argtypes = [C.c_int32, C.c_int64, C.c_int32]
+    v = lib.dgprobe_copy_gbps(device, nbytes, reps)
+    return v if v > 0 else None
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH/WRITE summary."""
     path = os.path.join(ROOT, "profiles", "stage_kernel_c2_traffic.txt")
@@ -159,6 +173,7 @@ def main():
     stage_sum64 = sum(out["fp64"]["stage_ms"])
     achieved64 = BYTES_PER_DOF["fp64"] * dof / (stage_sum64 * 1e-3) / 1e9
 
+    copy_gbps = copy_bandwidth()
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(m, bc, dt, args.cpu_budget)
@@ -189,6 +204,8 @@ def main():
         "rel_l2_observer_p_vs_fp64": rel_o,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS,
+                     "copy_gbps": copy_gbps,
+                     "frac_of_copy": achieved / copy_gbps if copy_gbps else None,
                      "traffic": measured_traffic(kernel_name) if args.n == 32 else None,
                      "kernel": kernel_name,
                      "stage_ms": mixed["stage_ms"],
