@@ -295,8 +295,7 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T>::kBlockSize), (StageWave
 
     // one batch of EB elements per block (one batch per block measured faster than persistent
     // blocks in both precisions); each XCD takes a contiguous range of batches
-    const int64_t bi = xcdRemap(blockIdx.x, gridDim.x);
-    {
+    const int64_t bi = xcdRemap(blockIdx.x, gridDim.x);  // the grid has one block per batch
     const int64_t e0 = bi * EB;
     const int nE = static_cast<int>((a.K - e0) < EB ? (a.K - e0) : EB);
     const bool active = tid < nE * NP;
@@ -571,7 +570,6 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T>::kBlockSize), (StageWave
                 res[4] += d[3] * d[3];
             }
         }
-    }
     }
     if constexpr (kFinal) {
         if (a.snap) {
