@@ -175,11 +175,13 @@ struct ClosureArgs {
     dgk::Physics<T> ph;
 };
 
-constexpr int kClosureFacesPerBlock = 4;
+// One wave per face, 16 faces per block: the network is staged in LDS once per 16 faces.
+constexpr int kClosureFacesPerBlock = 16;
+constexpr int kClosureBlock = 64 * kClosureFacesPerBlock;
 
-// Exterior states of the learned faces for one stage: one wave per face, four per block.
+// Exterior states of the learned faces for one stage.
 template <int DIM, int P, int NF, typename T>
-__global__ __launch_bounds__(256) void closureKernel(ClosureArgs<T> a) {
+__global__ __launch_bounds__(kClosureBlock) void closureKernel(ClosureArgs<T> a) {
     using S = dgk::Sizes<DIM, P>;
     using L = dgk::OpLayout<DIM, P>;
     constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC;
@@ -191,7 +193,7 @@ __global__ __launch_bounds__(256) void closureKernel(ClosureArgs<T> a) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t idx = static_cast<int64_t>(blockIdx.x) * kClosureFacesPerBlock + wave;
     const int64_t ef = a.faces[idx < a.nfaces ? idx : a.nfaces - 1];  // issued with the network loads
-    loadNet<NIN, NOUT, 256>(a.w, net, tid);
+    loadNet<NIN, NOUT, kClosureBlock>(a.w, net, tid);
     __syncthreads();
     if (idx >= a.nfaces) return;  // whole wave leaves; no block barrier follows
     float* sx = buf[wave];

@@ -363,8 +363,8 @@ void launchClosure(dg_context* c, const void* qin) {
     a.nfaces = c->nLearned;
     a.ph = makePhysics<T>(c->phys);
     const int64_t nb = (c->nLearned + dgc::kClosureFacesPerBlock - 1) / dgc::kClosureFacesPerBlock;
-    hipLaunchKernelGGL((dgc::closureKernel<DIM, P, NF, T>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, c->stream,
-                       a);
+    hipLaunchKernelGGL((dgc::closureKernel<DIM, P, NF, T>), dim3(static_cast<unsigned>(nb)), dim3(dgc::kClosureBlock), 0,
+                       c->stream, a);
 }
 
 template <int DIM, int P, int NF, typename T>
