@@ -71,9 +71,6 @@ struct dg_context {
     int* nonfinite = nullptr;
     int64_t* counter = nullptr;
     int64_t* batchSteps = nullptr;
-    unsigned int* ticket = nullptr;
-    double* resPartial2 = nullptr;
-    int64_t postBlocks = 1;
     int64_t ringSlots = 0;
     // pinned staging
     double *hT = nullptr, *hV = nullptr, *hRes = nullptr, *hObs = nullptr;
@@ -328,8 +325,6 @@ void launchPost(dg_context* c) {
     dgk::PostArgs p{};
     p.resPartial = c->resPartial;
     p.nblocks = c->nblocks;
-    p.resPartial2 = c->resPartial2;
-    p.ticket = c->ticket;
     p.invN = 1.0 / static_cast<double>(c->K * c->Np);
     p.obsOffsets = c->dObsOff;
     p.obsIdx = c->dObsIdx;
@@ -347,8 +342,8 @@ void launchPost(dg_context* c) {
     p.stepCounter = c->counter;
     p.batchSteps = c->batchSteps;
     p.computeResiduals = c->curResid;
-    hipLaunchKernelGGL((dgk::postKernel<T, COMP>), dim3(static_cast<unsigned>(c->postBlocks)), dim3(256), 0, c->stream,
-                       p, sourceArgs(c), static_cast<T*>(c->u), static_cast<T*>(COMP ? c->ulo : nullptr));
+    hipLaunchKernelGGL((dgk::postKernel<T, COMP>), dim3(1), dim3(dgk::kPostBlock), 0, c->stream, p, sourceArgs(c),
+                       static_cast<T*>(c->u), static_cast<T*>(COMP ? c->ulo : nullptr));
 }
 
 template <int DIM, int P, int NF, typename T>
@@ -707,8 +702,6 @@ void destroyContext(dg_context* c) {
     devFree(c->nonfinite);
     devFree(c->counter);
     devFree(c->batchSteps);
-    devFree(c->ticket);
-    devFree(c->resPartial2);
     devFree(c->dLearned);
     devFree(c->ghost);
     devFree(c->dClosureW);
@@ -854,10 +847,6 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         c->nonfinite = devAlloc<int>(c, 1);
         c->counter = devAlloc<int64_t>(c, 1);
         c->batchSteps = devAlloc<int64_t>(c, 1);
-        c->ticket = devAlloc<unsigned int>(c, 1);
-        HIPCHK(hipMemset(c->ticket, 0, sizeof(unsigned int)));
-        c->postBlocks = std::max<int64_t>(1, std::min<int64_t>(128, c->nblocks / 128));
-        c->resPartial2 = devAlloc<double>(c, static_cast<size_t>(c->postBlocks) * 5);
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hT), sizeof(double) * c->ringSlots));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hRes), sizeof(double) * c->ringSlots * 5));
         c->tablesDirty = true;

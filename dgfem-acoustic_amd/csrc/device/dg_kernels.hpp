@@ -1381,8 +1381,6 @@ __global__ __launch_bounds__(256) void sourceKernel(SourceArgs s, const int64_t*
 struct PostArgs {
     const double* resPartial;   // [nblocks][5] from the final stage
     int64_t nblocks;
-    double* resPartial2;        // [gridDim][5]
-    unsigned int* ticket;       // last-block ticket (returns to 0 after each step)
     double invN;                // 1/(K*Np)
     const int64_t* obsOffsets;  // [nobs+1]
     const int64_t* obsIdx;      // element-node indices
@@ -1402,77 +1400,86 @@ struct PostArgs {
     int computeResiduals;
 };
 
-// End-of-step work: residuals (src/solver.cpp:518-547, deterministic two-level sum),
-// inverse-distance-weighted observers (src/solver.cpp:552-573), then the source overwrite of
-// the next step of the batch, and the step counter advance. The last block to finish the
-// first-level sums does everything after it.
+// End-of-step work in one block of kPostBlock threads: residuals (src/solver.cpp:518-547;
+// thread t sums the stage blocks t, t + kPostBlock, ... in that order, then a fixed-order tree),
+// inverse-distance-weighted observers (src/solver.cpp:552-573; one wave per node-list segment
+// of <= 256 entries), then the source overwrite of the next step of the batch and the step
+// counter advance. Loads of a phase are issued together from clamped addresses.
+constexpr int kPostBlock = 1024;
+
 template <typename T, bool COMP>
-__global__ __launch_bounds__(256) void postKernel(PostArgs p, SourceArgs s, T* __restrict__ u, T* __restrict__ ulo) {
-    __shared__ double red[256][5];
-    __shared__ int amLast;
-    const int tid = threadIdx.x;
+__global__ __launch_bounds__(kPostBlock) void postKernel(PostArgs p, SourceArgs s, T* __restrict__ u,
+                                                         T* __restrict__ ulo) {
+    constexpr int NW = kPostBlock / 64;
+    __shared__ double red[NW][5];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t slot = *p.stepCounter;
+    // residual partial sums of this thread, 8 stage blocks per round trip
     double acc[5] = {0, 0, 0, 0, 0};
     if (p.computeResiduals) {
-        const int64_t chunk = (p.nblocks + gridDim.x - 1) / gridDim.x;
-        const int64_t b0 = blockIdx.x * chunk;
-        const int64_t b1 = (b0 + chunk < p.nblocks) ? b0 + chunk : p.nblocks;
-        for (int64_t b = b0 + tid; b < b1; b += 256)
-            for (int k = 0; k < 5; ++k) acc[k] += p.resPartial[b * 5 + k];
-    }
-    for (int k = 0; k < 5; ++k) red[tid][k] = acc[k];
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w)
-            for (int k = 0; k < 5; ++k) red[tid][k] += red[tid + w][k];
-        __syncthreads();
-    }
-    if (tid < 5) p.resPartial2[blockIdx.x * 5 + tid] = red[0][tid];
-    // observer segments: one entry per thread, fixed-order tree per segment
-    for (int sg = blockIdx.x; sg < p.nseg; sg += gridDim.x) {
-        __syncthreads();
-        double ob[5] = {0, 0, 0, 0, 0};
-        const int64_t x = p.segBegin[2 * sg] + tid;
-        if (x < p.segBegin[2 * sg + 1]) {
-            const double w = p.obsW[x];
-            const int64_t base = p.obsIdx[x] * p.nf;
-            for (int k = 0; k < 4; ++k) {
-                double v = static_cast<double>(u[base + k]);
-                if constexpr (COMP) v += static_cast<double>(ulo[base + k]);
-                ob[k] = v * w;
+        constexpr int U = 8;
+        for (int64_t b0 = tid; b0 < p.nblocks; b0 += U * kPostBlock) {
+            double v[U][5];
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                const int64_t b = b0 + static_cast<int64_t>(q) * kPostBlock;
+                const int64_t bc = b < p.nblocks ? b : p.nblocks - 1;
+#pragma unroll
+                for (int k = 0; k < 5; ++k) v[q][k] = p.resPartial[bc * 5 + k];
             }
-            ob[4] = w;
+#pragma unroll
+            for (int q = 0; q < U; ++q)
+                if (b0 + static_cast<int64_t>(q) * kPostBlock < p.nblocks)
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) acc[k] += v[q][k];
         }
-        for (int k = 0; k < 5; ++k) red[tid][k] = ob[k];
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if (tid < w)
-                for (int k = 0; k < 5; ++k) red[tid][k] += red[tid + w][k];
-            __syncthreads();
-        }
-        if (tid < 5) p.segPartial[sg * 5 + tid] = red[0][tid];
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) amLast = (atomicAdd(p.ticket, 1u) == gridDim.x - 1) ? 1 : 0;
-    __syncthreads();
-    if (!amLast) return;
-    __threadfence();
-    // second level: one first-level partial per thread, then a fixed-order tree
+    // observer segments: one wave per segment, 4 entries per lane
+    for (int sg = wave; sg < p.nseg; sg += NW) {
+        const int64_t x0 = p.segBegin[2 * sg], x1 = p.segBegin[2 * sg + 1];
+        int64_t base[4];
+        double w[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int64_t x = x0 + lane + 64 * q;
+            const int64_t xc = x < x1 ? x : x0;
+            base[q] = p.obsIdx[xc] * p.nf;
+            w[q] = x < x1 ? p.obsW[xc] : 0.0;
+        }
+        double vq[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                vq[q][k] = static_cast<double>(u[base[q] + k]);
+                if constexpr (COMP) vq[q][k] += static_cast<double>(ulo[base[q] + k]);
+            }
+        double ob[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) ob[k] += vq[q][k] * w[q];
+            ob[4] += w[q];
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            for (int off = 32; off > 0; off >>= 1) ob[k] += __shfl_down(ob[k], off, 64);
+        if (lane == 0)
+#pragma unroll
+            for (int k = 0; k < 5; ++k) p.segPartial[sg * 5 + k] = ob[k];
+    }
+    // residuals: wave trees, then the waves in order
+#pragma unroll
     for (int k = 0; k < 5; ++k) {
-        double v = 0.0;
-        for (unsigned b = tid; b < gridDim.x; b += 256)
-            v += __hip_atomic_load(&p.resPartial2[b * 5 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        red[tid][k] = v;
+        double v = acc[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[wave][k] = v;
     }
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w)
-            for (int k = 0; k < 5; ++k) red[tid][k] += red[tid + w][k];
-        __syncthreads();
-    }
+    __syncthreads();  // also orders the segment partials (global) and every read of u above
     if (tid < 5) {
-        const double v = red[0][tid] * p.invN;
+        double v = 0.0;
+        for (int w = 0; w < NW; ++w) v += red[w][tid];
+        v *= p.invN;
         p.resRing[slot * 5 + tid] = v;
         if (!isfinite(v)) atomicCAS(p.nonfinite, -1, static_cast<int>(slot));
     }
@@ -1480,17 +1487,12 @@ __global__ __launch_bounds__(256) void postKernel(PostArgs p, SourceArgs s, T* _
     if (tid < p.nobs) {
         double ob[5] = {0, 0, 0, 0, 0};
         for (int sg = p.obsSeg0[tid]; sg < p.obsSeg0[tid + 1]; ++sg)
-            for (int k = 0; k < 5; ++k)
-                ob[k] += __hip_atomic_load(&p.segPartial[sg * 5 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int k = 0; k < 5; ++k) ob[k] += p.segPartial[sg * 5 + k];
         for (int k = 0; k < 4; ++k) p.obsRing[(slot * p.nobs + tid) * 4 + k] = ob[k] / ob[4];
     }
-    __syncthreads();
     if (slot + 1 < *p.batchSteps) applySources<T, COMP>(s, slot + 1, u, ulo);
     __syncthreads();
-    if (tid == 0) {
-        *p.ticket = 0u;
-        *p.stepCounter = slot + 1;
-    }
+    if (tid == 0) *p.stepCounter = slot + 1;
 }
 
 [[maybe_unused]] static __global__ void setCounterKernel(int64_t* counter, int64_t* batchSteps, int64_t nsteps, int* flag) {
