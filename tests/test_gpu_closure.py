@@ -182,8 +182,8 @@ def _learned_case3d(n, p, seed=6):
     return m, bc
 
 
-@pytest.mark.parametrize("p,precision,v0", [(2, dg.DG_FP64, (40.0, -25.0, 15.0)), (3, dg.DG_FP64, (0.0, 0.0, 0.0)),
-                                            (3, dg.DG_MIXED, (30.0, 0.0, 0.0))])
+@pytest.mark.parametrize("p,precision,v0", [(1, dg.DG_FP64, (-10.0, 20.0, 5.0)), (2, dg.DG_FP64, (40.0, -25.0, 15.0)),
+                                            (3, dg.DG_FP64, (0.0, 0.0, 0.0)), (3, dg.DG_MIXED, (30.0, 0.0, 0.0))])
 def test_rhs_with_learned_faces_matches_nodal_restatement_3d(p, precision, v0):
     """Tetrahedra: the closure kernel's exterior states (3D face frame, two tangential
     components) against the oracle's, through the whole right-hand side. v0 = 0 exercises the
