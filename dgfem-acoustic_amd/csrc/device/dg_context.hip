@@ -49,7 +49,7 @@ struct dg_context {
     dg_options opt{};
     // device state
     void *u = nullptr, *ulo = nullptr, *qa = nullptr, *qb = nullptr, *acc = nullptr, *snap = nullptr,
-         *snaplo = nullptr, *geo = nullptr, *fgeo = nullptr, *ops = nullptr;
+         *geo = nullptr, *fgeo = nullptr, *ops = nullptr;
     int32_t *nbr = nullptr, *iops = nullptr;
     uint8_t* code = nullptr;
     double* resPartial = nullptr;
@@ -155,7 +155,6 @@ void launchStage(dg_context* c, int stage, const void* qin, void* qout, void* uo
     a.ops = static_cast<const T*>(c->ops);
     a.iops = c->iops;
     a.snap = snap ? static_cast<const T*>(c->snap) : nullptr;
-    a.snaplo = (snap && COMP) ? static_cast<const T*>(c->snaplo) : nullptr;
     a.resPartial = c->resPartial;
     a.K = c->K;
     a.dt = static_cast<T>(c->curDt);
@@ -214,7 +213,6 @@ void launchStageWave(dg_context* c, int stage, const void* qin, void* qout, void
     a.ops = static_cast<const T*>(c->ops);
     a.iops = c->iops;
     a.snap = snap ? static_cast<const T*>(c->snap) : nullptr;
-    a.snaplo = (snap && COMP) ? static_cast<const T*>(c->snaplo) : nullptr;
     a.resPartial = c->resPartial;
     a.K = c->K;
     a.dt = static_cast<T>(c->curDt);
@@ -242,7 +240,6 @@ void launchStageMM(dg_context* c, int stage, const void* qin, void* qout, void* 
     a.ops = static_cast<const T*>(c->ops);
     a.iops = c->iops;
     a.snap = snap ? static_cast<const T*>(c->snap) : nullptr;
-    a.snaplo = (snap && COMP) ? static_cast<const T*>(c->snaplo) : nullptr;
     a.resPartial = c->resPartial;
     a.K = c->K;
     a.dt = static_cast<T>(c->curDt);
@@ -688,7 +685,7 @@ void destroyContext(dg_context* c) {
     if (c->device >= 0) (void)hipSetDevice(c->device);
     releaseGraphs(c);
     freeTables(c);
-    for (void* p : {c->u, c->ulo, c->qa, c->qb, c->acc, c->snap, c->snaplo, c->geo, c->fgeo, c->ops, c->afrag})
+    for (void* p : {c->u, c->ulo, c->qa, c->qb, c->acc, c->snap, c->geo, c->fgeo, c->ops, c->afrag})
         devFree(p);
     devFree(c->nbr);
     devFree(c->iops);
@@ -787,11 +784,10 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         c->qa = alloc(nstate);
         c->qb = alloc(nstate);
         c->acc = alloc(nstate);
+        // the residual snapshot holds the state's leading word only (mixed precision: the FP32
+        // value nearest the compensated state, so a residual term is off by <= 2^-24 |snapshot|)
         c->snap = alloc(nstate);
-        if (c->precision == DG_MIXED) {
-            c->ulo = alloc(nstate);
-            c->snaplo = alloc(nstate);
-        }
+        if (c->precision == DG_MIXED) c->ulo = alloc(nstate);
         HIPCHK(hipMemset(c->u, 0, nstate * c->tsize));
         if (c->ulo) HIPCHK(hipMemset(c->ulo, 0, nstate * c->tsize));
         // geometry: [K][dim*dim] dxi_r/dx_c
@@ -1070,8 +1066,6 @@ dg_status dg_run(dg_context* c, const dg_run_params* prm, dg_step_fn on_step, dg
             const int64_t n = end - done;
             if (isSave[done]) {
                 HIPCHK(hipMemcpyAsync(c->snap, c->u, stateN * c->tsize, hipMemcpyDeviceToDevice, c->stream));
-                if (c->snaplo)
-                    HIPCHK(hipMemcpyAsync(c->snaplo, c->ulo, stateN * c->tsize, hipMemcpyDeviceToDevice, c->stream));
                 if (on_save) {
                     HIPCHK(hipStreamSynchronize(c->stream));
                     std::vector<double> dev;

@@ -71,8 +71,7 @@ struct StageArgs {
     const uint8_t* __restrict__ code;  // [K][NFC]
     const T* __restrict__ ops;
     const int32_t* __restrict__ iops;
-    const T* __restrict__ snap;       // snapshot (hi) for residuals, RK4/Euler final stage
-    const T* __restrict__ snaplo;     // snapshot compensation or null
+    const T* __restrict__ snap;       // residual snapshot (leading word of the state), RK4/Euler final stage
     double* __restrict__ resPartial;  // [gridDim][5]
     int64_t K;
     T dt;
@@ -218,8 +217,48 @@ __device__ inline void twoSum(T a, T b, T& s, T& e) {
     e = (a - (s - bb)) + (b - bb);
 }
 
-// FP32 with 4 fields and 10..20 nodes per element asks for 7 waves per SIMD (keeps the
-// single-batch kernel at <= 72 VGPRs; measured faster); other variants are left to the compiler.
+// A double moved between lanes of a row of 16 by a DPP control (VALU; no LDS traffic).
+template <int CTRL>
+__device__ inline double dppMove(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = static_cast<uint32_t>(
+        __builtin_amdgcn_mov_dpp(static_cast<int>(static_cast<uint32_t>(b)), CTRL, 0xF, 0xF, false));
+    const uint32_t hi = static_cast<uint32_t>(
+        __builtin_amdgcn_mov_dpp(static_cast<int>(static_cast<uint32_t>(b >> 32)), CTRL, 0xF, 0xF, false));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// Sum over each row of 16 lanes, left in every lane of the row: partner lanes are paired
+// symmetrically at every level (neighbour, pair of pairs, mirrored halves, mirrored row), so
+// all lanes of a row compute the same value in the same order. Every lane of the wave must be
+// active.
+__device__ inline double rowSum16(double x) {
+    x += dppMove<0xB1>(x);   // quad_perm [1,0,3,2]
+    x += dppMove<0x4E>(x);   // quad_perm [2,3,0,1]
+    x += dppMove<0x141>(x);  // row_half_mirror
+    x += dppMove<0x140>(x);  // row_mirror
+    return x;
+}
+
+// Residual partial sums of a block in a fixed order: rows of 16 threads by rowSum16, then the
+// rows in ascending order; thread k < 5 writes out[k]. sRed holds kBlock/16 rows. Called by
+// every thread of the block.
+template <int kBlock>
+__device__ inline void blockResiduals(const double (&res)[5], double (*sRed)[5], double* __restrict__ out) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const double v = rowSum16(res[k]);
+        if ((tid & 15) == 0) sRed[tid >> 4][k] = v;
+    }
+    __syncthreads();
+    if (tid < 5) {
+        double s = 0.0;
+        for (int r = 0; r < kBlock / 16; ++r) s += sRed[r][tid];
+        out[tid] = s;
+    }
+}
+
 template <int DIM, int P, int NF, typename T>
 struct StageWaves {
     static constexpr int NP = Sizes<DIM, P>::NP;
@@ -287,7 +326,7 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T>::kBlockSize), (StageWave
     __shared__ __align__(16) uint8_t sCd8[EB * NFC]; // per (element, face): neighbour face | perm << 2
     __shared__ __align__(16) T sOps[kOpsInLds ? L::kTotal : 1];
     __shared__ int32_t sI[L::kITotal];
-    __shared__ double sRed[kBlock / 64][5];
+    __shared__ double sRed[kBlock / 16][5];
 
     const int tid = threadIdx.x;
     const T* ops = kOpsInLds ? sOps : a.ops;
@@ -345,9 +384,9 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T>::kBlockSize), (StageWave
     // RK operands of this thread's node (consumed after the contractions). The compensation
     // word enters only the final accumulation: stage inputs are the FP32 u + c k (measured: same
     // error against FP64 after 2000 steps, 4 B/DOF less per stage).
-    T pu[NF], pacc[NF], plo[NF], ps[NF], psl[NF];
+    T pu[NF], pacc[NF], plo[NF], ps[NF];
 #pragma unroll
-    for (int k = 0; k < NF; ++k) pu[k] = pacc[k] = plo[k] = ps[k] = psl[k] = T(0);
+    for (int k = 0; k < NF; ++k) pu[k] = pacc[k] = plo[k] = ps[k] = T(0);
     if constexpr (kNeedU) {
         loadNodeNT<NF>(a.u + ldBase, pu);
         loadNodeNT<NF>(a.acc + ldBase, pacc);
@@ -415,7 +454,6 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T>::kBlockSize), (StageWave
     // snapshot for the residuals, after the gathers (its wait then coincides with theirs)
     if constexpr (kFinal) {
         loadNodeNT<NF>((a.snap ? a.snap : a.qin) + ldBase, ps);
-        if constexpr (COMP) loadNodeNT<NF>((a.snaplo ? a.snaplo : a.qin) + ldBase, psl);
     }
 #pragma unroll
     for (int c = 0; c < kFIt; ++c) {
@@ -557,7 +595,6 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T>::kBlockSize), (StageWave
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     sv[k] = static_cast<double>(ps[k]);
-                    if constexpr (COMP) sv[k] += static_cast<double>(psl[k]);
                     d[k] = (static_cast<double>(unew[k]) + lo[k]) - sv[k];
                 }
                 const double ic2 = static_cast<double>(a.ph.ic2);
@@ -573,19 +610,7 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T>::kBlockSize), (StageWave
     }
     if constexpr (kFinal) {
         if (a.snap) {
-            // block reduction of the residual partial sums (deterministic order)
-#pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                double v = res[k];
-                for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-                if ((tid & 63) == 0) sRed[tid >> 6][k] = v;
-            }
-            __syncthreads();
-            if (tid < 5) {
-                double s = 0.0;
-                for (int w = 0; w < kBlock / 64; ++w) s += sRed[w][tid];
-                a.resPartial[static_cast<int64_t>(blockIdx.x) * 5 + tid] = s;
-            }
+            blockResiduals<kBlock>(res, sRed, a.resPartial + static_cast<int64_t>(blockIdx.x) * 5);
         }
     }
 }
@@ -672,7 +697,7 @@ __global__ __launch_bounds__(kBlock) void stageKernelWave(
     __shared__ __align__(16) T sR[4][4 * NP * NF];    // per wave: rhs transpose
     __shared__ T sJ[4][4 * JJ];
     __shared__ T sN[4][4 * NFC * 4];
-    __shared__ double sRed[kBlock / 64][5];
+    __shared__ double sRed[kBlock / 16][5];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t blk = xcdRemap(blockIdx.x, gridDim.x);
@@ -822,7 +847,7 @@ __global__ __launch_bounds__(kBlock) void stageKernelWave(
             }
             const V4 q0 = reinterpret_cast<const V4*>(Q)[node];
             const T qv[4] = {q0.x, q0.y, q0.z, q0.w};
-            V4 pu[1], pacc[1], plo[1], ps[1], psl[1];
+            V4 pu[1], pacc[1], plo[1], ps[1];
             const int cc = c;
             c = 0;
             if (needU) {
@@ -832,10 +857,7 @@ __global__ __launch_bounds__(kBlock) void stageKernelWave(
             plo[0] = V4{T(0), T(0), T(0), T(0)};
             if constexpr (COMP)
                 if (isFinal) plo[0] = reinterpret_cast<const V4*>(a.ulo)[off];
-            if (isFinal && a.snap) {
-                ps[0] = reinterpret_cast<const V4*>(a.snap)[off];
-                if constexpr (COMP) psl[0] = reinterpret_cast<const V4*>(a.snaplo)[off];
-            }
+            if (isFinal && a.snap) ps[0] = reinterpret_cast<const V4*>(a.snap)[off];
             const T uv4[4] = {pu[c].x, pu[c].y, pu[c].z, pu[c].w};
             const T ac4[4] = {pacc[c].x, pacc[c].y, pacc[c].z, pacc[c].w};
             T lo4[4] = {T(0), T(0), T(0), T(0)};
@@ -882,14 +904,10 @@ __global__ __launch_bounds__(kBlock) void stageKernelWave(
                 }
                 if (a.snap) {
                     const T s4[4] = {ps[c].x, ps[c].y, ps[c].z, ps[c].w};
-                    T sl4[4] = {T(0), T(0), T(0), T(0)};
-                    if constexpr (COMP) {
-                        sl4[0] = psl[c].x; sl4[1] = psl[c].y; sl4[2] = psl[c].z; sl4[3] = psl[c].w;
-                    }
                     double d[4], sv0 = 0.0, un0 = 0.0;
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
-                        const double sv = static_cast<double>(s4[k]) + static_cast<double>(sl4[k]);
+                        const double sv = static_cast<double>(s4[k]);
                         const double un = static_cast<double>(hi[k]) + static_cast<double>(lw[k]);
                         d[k] = un - sv;
                         if (k == 0) { sv0 = sv; un0 = un; }
@@ -907,20 +925,7 @@ __global__ __launch_bounds__(kBlock) void stageKernelWave(
         }
         waveSync();
     }
-    if (isFinal && a.snap) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            double v = res[k];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-            if (lane == 0) sRed[wave][k] = v;
-        }
-        __syncthreads();
-        if (tid < 5) {
-            double sum = 0.0;
-            for (int w = 0; w < kBlock / 64; ++w) sum += sRed[w][tid];
-            a.resPartial[static_cast<int64_t>(blockIdx.x) * 5 + tid] = sum;
-        }
-    }
+    if (isFinal && a.snap) blockResiduals<kBlock>(res, sRed, a.resPartial + static_cast<int64_t>(blockIdx.x) * 5);
 }
 
 // --------------------------------------------------- block matrix-core stage kernel ----
@@ -984,7 +989,7 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
     __shared__ int32_t sNb[EB * NFC];
     __shared__ int32_t sCd[EB * NFC];
     __shared__ int32_t sI[L::kITotal];
-    __shared__ double sRed[kBlock / 64][5];
+    __shared__ double sRed[kBlock / 16][5];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t blk = xcdRemap(blockIdx.x, gridDim.x);
@@ -1024,7 +1029,7 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
     // RK operands of this lane's update items (node-major over the wave's 4 elements)
     int64_t uNode[NIT];
     bool uOk[NIT];
-    T pu[NIT][NF], pacc[NIT][NF], plo[NIT][NF], ps[NIT][NF], psl[NIT][NF];
+    T pu[NIT][NF], pacc[NIT][NF], plo[NIT][NF], ps[NIT][NF];
 #pragma unroll
     for (int c = 0; c < NIT; ++c) {
         const int it = lane + 64 * c;
@@ -1032,7 +1037,7 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
         uOk[c] = it < 4 * NP && el < nEw;
         uNode[c] = uOk[c] ? (e0 + 4 * wave + el) * NP + i : e0 * NP;
 #pragma unroll
-        for (int k = 0; k < NF; ++k) pu[c][k] = pacc[c][k] = plo[c][k] = ps[c][k] = psl[c][k] = T(0);
+        for (int k = 0; k < NF; ++k) pu[c][k] = pacc[c][k] = plo[c][k] = ps[c][k] = T(0);
         if constexpr (kNeedU) {
             loadNodeNT<NF>(a.u + uNode[c] * NF, pu[c]);
             loadNodeNT<NF>(a.acc + uNode[c] * NF, pacc[c]);
@@ -1107,7 +1112,6 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
 #pragma unroll
         for (int c = 0; c < NIT; ++c) {
             loadNodeNT<NF>((a.snap ? a.snap : a.qin) + uNode[c] * NF, ps[c]);
-            if constexpr (COMP) loadNodeNT<NF>((a.snaplo ? a.snaplo : a.qin) + uNode[c] * NF, psl[c]);
         }
     }
     // A fragments of this lane and the wave's row tiles (constant operators), stored
@@ -1276,7 +1280,6 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     sv[k] = static_cast<double>(ps[c][k]);
-                    if constexpr (COMP) sv[k] += static_cast<double>(psl[c][k]);
                     d[k] = (static_cast<double>(unew[k]) + static_cast<double>(lw[k])) - sv[k];
                 }
                 const double ic2 = static_cast<double>(a.ph.ic2);
@@ -1291,20 +1294,7 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
         }
     }
     if constexpr (kFinal) {
-        if (a.snap) {
-#pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                double v = res[k];
-                for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-                if (lane == 0) sRed[wave][k] = v;
-            }
-            __syncthreads();
-            if (tid < 5) {
-                double sum = 0.0;
-                for (int w = 0; w < kBlock / 64; ++w) sum += sRed[w][tid];
-                a.resPartial[static_cast<int64_t>(blockIdx.x) * 5 + tid] = sum;
-            }
-        }
+        if (a.snap) blockResiduals<kBlock>(res, sRed, a.resPartial + static_cast<int64_t>(blockIdx.x) * 5);
     }
 }
 
