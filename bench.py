@@ -157,7 +157,7 @@ def main():
         out[name] = {"wall_s": wall, "device_ms": r.device_ms, "ms_per_step": 1e3 * wall / args.steps,
                      "value": dof * args.steps / wall, "stage_ms": stage_ms, "source_ms": ms[1], "post_ms": ms[2]}
         print(f"# {name}: {out[name]['value']:.4e} DOF-updates/s, {out[name]['ms_per_step']:.4f} ms/step, "
-              f"stage kernels {['%.4f' % x for x in stage_ms]} ms", file=sys.stderr, flush=True)
+              f"stage kernels {['%.4f' % x for x in stage_ms]} ms, post {ms[2]:.4f} ms", file=sys.stderr, flush=True)
         s.close()
 
     p64, o64 = states["fp64"]

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../../include/dgfem_amd.h"
@@ -60,8 +61,9 @@ struct dg_context {
     std::vector<TableSource> srcTables;            // per source (fn null for monopoles)
     std::vector<std::vector<int64_t>> obsNodes;
     std::vector<std::vector<double>> obsW;
-    int64_t *dSrcOff = nullptr, *dSrcIdx = nullptr, *dObsOff = nullptr, *dObsIdx = nullptr, *dSegBegin = nullptr;
-    int32_t *dSegObs = nullptr, *dObsSeg0 = nullptr;
+    int64_t *dSrcNodes = nullptr, *dSrcCoverOff = nullptr, *dObsIdx = nullptr;
+    int32_t *dSrcLast = nullptr, *dSrcCover = nullptr, *dObsSeg0 = nullptr;
+    int64_t nSrcNodes = 0;
     double* dSegPartial = nullptr;
     int nseg = 0;
     double *dSrcParams = nullptr, *dObsW = nullptr;
@@ -182,8 +184,11 @@ void launchStage(dg_context* c, int stage, const void* qin, void* qout, void* uo
 
 dgk::SourceArgs sourceArgs(dg_context* c) {
     dgk::SourceArgs s{};
-    s.nodeOffsets = c->dSrcOff;
-    s.nodeIdx = c->dSrcIdx;
+    s.nodes = c->dSrcNodes;
+    s.lastSrc = c->dSrcLast;
+    s.coverOff = c->dSrcCoverOff;
+    s.coverSrc = c->dSrcCover;
+    s.nnodes = c->nSrcNodes;
     s.params = c->dSrcParams;
     s.tTable = c->dT;
     s.vTable = c->dV;
@@ -323,11 +328,8 @@ void launchPost(dg_context* c) {
     p.resPartial = c->resPartial;
     p.nblocks = c->nblocks;
     p.invN = 1.0 / static_cast<double>(c->K * c->Np);
-    p.obsOffsets = c->dObsOff;
     p.obsIdx = c->dObsIdx;
     p.obsW = c->dObsW;
-    p.segBegin = c->dSegBegin;
-    p.segObs = c->dSegObs;
     p.obsSeg0 = c->dObsSeg0;
     p.segPartial = c->dSegPartial;
     p.nseg = c->nseg;
@@ -496,21 +498,14 @@ void releaseGraphs(dg_context* c) {
 }
 
 void freeTables(dg_context* c) {
-    devFree(c->dSrcOff);
-    devFree(c->dSrcIdx);
-    devFree(c->dSrcParams);
-    devFree(c->dObsOff);
-    devFree(c->dObsIdx);
-    devFree(c->dObsW);
-    devFree(c->dSegBegin);
-    devFree(c->dSegObs);
-    devFree(c->dObsSeg0);
-    devFree(c->dSegPartial);
-    c->dSegBegin = nullptr;
-    c->dSegObs = c->dObsSeg0 = nullptr;
-    c->dSegPartial = nullptr;
-    c->dSrcOff = c->dSrcIdx = c->dObsOff = c->dObsIdx = nullptr;
-    c->dSrcParams = c->dObsW = nullptr;
+    for (void* p : {static_cast<void*>(c->dSrcNodes), static_cast<void*>(c->dSrcLast), static_cast<void*>(c->dSrcCoverOff),
+                    static_cast<void*>(c->dSrcCover), static_cast<void*>(c->dSrcParams), static_cast<void*>(c->dObsIdx),
+                    static_cast<void*>(c->dObsW), static_cast<void*>(c->dObsSeg0), static_cast<void*>(c->dSegPartial)})
+        devFree(p);
+    c->dSrcNodes = c->dSrcCoverOff = c->dObsIdx = nullptr;
+    c->dSrcLast = c->dSrcCover = c->dObsSeg0 = nullptr;
+    c->dSrcParams = c->dObsW = c->dSegPartial = nullptr;
+    c->nSrcNodes = 0;
 }
 
 template <typename V>
@@ -524,52 +519,64 @@ void syncTables(dg_context* c) {
     if (!c->tablesDirty) return;
     releaseGraphs(c);
     freeTables(c);
-    std::vector<int64_t> off{0}, idx;
+    // source table: the distinct nodes covered by a source (ascending), each with the sources
+    // covering it in definition order and the last of them
+    std::vector<std::pair<int64_t, int32_t>> cover;
     std::vector<double> prm;
     for (size_t s = 0; s < c->srcNodes.size(); ++s) {
-        idx.insert(idx.end(), c->srcNodes[s].begin(), c->srcNodes[s].end());
-        off.push_back(static_cast<int64_t>(idx.size()));
+        for (int64_t n : c->srcNodes[s]) cover.emplace_back(n, static_cast<int32_t>(s));
         prm.insert(prm.end(), c->srcParams[s].begin(), c->srcParams[s].end());
     }
-    c->dSrcOff = uploadVec(c, off);
-    c->dSrcIdx = uploadVec(c, idx);
-    c->dSrcParams = uploadVec(c, prm);
-    std::vector<int64_t> ooff{0}, oidx;
-    std::vector<double> ow;
-    for (size_t o = 0; o < c->obsNodes.size(); ++o) {
-        oidx.insert(oidx.end(), c->obsNodes[o].begin(), c->obsNodes[o].end());
-        ow.insert(ow.end(), c->obsW[o].begin(), c->obsW[o].end());
-        ooff.push_back(static_cast<int64_t>(oidx.size()));
+    std::sort(cover.begin(), cover.end());
+    cover.erase(std::unique(cover.begin(), cover.end()), cover.end());
+    std::vector<int64_t> srcNodes, coverOff{0};
+    std::vector<int32_t> lastSrc, coverSrc;
+    for (size_t k = 0; k < cover.size(); ++k) {
+        if (k > 0 && cover[k].first != cover[k - 1].first) coverOff.push_back(static_cast<int64_t>(coverSrc.size()));
+        if (k == 0 || cover[k].first != cover[k - 1].first) srcNodes.push_back(cover[k].first);
+        coverSrc.push_back(cover[k].second);
     }
-    c->dObsOff = uploadVec(c, ooff);
+    if (!cover.empty()) coverOff.push_back(static_cast<int64_t>(coverSrc.size()));
+    for (size_t i = 0; i < srcNodes.size(); ++i) lastSrc.push_back(coverSrc[coverOff[i + 1] - 1]);
+    c->nSrcNodes = static_cast<int64_t>(srcNodes.size());
+    // entry 0 of every table is valid memory for the clamped loads of the kernels
+    if (srcNodes.empty()) {
+        srcNodes.push_back(0);
+        lastSrc.push_back(0);
+        coverSrc.push_back(0);
+    }
+    while (coverOff.size() < 2) coverOff.push_back(0);
+    prm.resize(std::max<size_t>(prm.size(), 5), 0.0);
+    c->dSrcNodes = uploadVec(c, srcNodes);
+    c->dSrcLast = uploadVec(c, lastSrc);
+    c->dSrcCoverOff = uploadVec(c, coverOff);
+    c->dSrcCover = uploadVec(c, coverSrc);
+    c->dSrcParams = uploadVec(c, prm);
+    // observer node lists in segments of 256 entries, each padded with its first node at weight 0
+    // (one all-padding segment of node 0 when no observer has a node)
+    std::vector<int64_t> oidx;
+    std::vector<double> ow;
+    std::vector<int32_t> obsSeg0{0};
+    int nseg = 0;
+    for (size_t o = 0; o < c->obsNodes.size(); ++o) {
+        const std::vector<int64_t>& nd = c->obsNodes[o];
+        const std::vector<double>& w = c->obsW[o];
+        for (size_t b = 0; b < nd.size(); b += 256, ++nseg)
+            for (size_t q = 0; q < 256; ++q) {
+                const bool in = b + q < nd.size();
+                oidx.push_back(nd[in ? b + q : b]);
+                ow.push_back(in ? w[b + q] : 0.0);
+            }
+        obsSeg0.push_back(nseg);
+    }
+    c->nseg = nseg;
+    if (oidx.empty()) {
+        oidx.assign(256, 0);
+        ow.assign(256, 0.0);
+    }
+    while (obsSeg0.size() < 2) obsSeg0.push_back(nseg);
     c->dObsIdx = uploadVec(c, oidx);
     c->dObsW = uploadVec(c, ow);
-    // observer node lists split into segments of at most 256 entries
-    std::vector<int64_t> segBegin;
-    std::vector<int32_t> segObs, obsSeg0{0};
-    for (size_t o = 0; o < c->obsNodes.size(); ++o) {
-        for (int64_t b = ooff[o]; b < ooff[o + 1]; b += 256) {
-            segBegin.push_back(b);
-            segObs.push_back(static_cast<int32_t>(o));
-        }
-        obsSeg0.push_back(static_cast<int32_t>(segBegin.size()));
-    }
-    c->nseg = static_cast<int>(segBegin.size());
-    segBegin.push_back(oidx.size());
-    // a segment ends where the next begins or at the end of its observer's list
-    std::vector<int64_t> segEndFix(segBegin);
-    for (int sgi = 0; sgi < c->nseg; ++sgi) {
-        int o = segObs[sgi];
-        segEndFix[sgi + 1] = std::min<int64_t>(segBegin[sgi] + 256, ooff[o + 1]);
-    }
-    // store begin/end pairs as consecutive entries: begin[s], end[s] == begin of the next slot
-    std::vector<int64_t> segBE(2 * std::max(1, c->nseg));
-    for (int sgi = 0; sgi < c->nseg; ++sgi) {
-        segBE[2 * sgi] = segBegin[sgi];
-        segBE[2 * sgi + 1] = segEndFix[sgi + 1];
-    }
-    c->dSegBegin = uploadVec(c, segBE);
-    c->dSegObs = uploadVec(c, segObs);
     c->dObsSeg0 = uploadVec(c, obsSeg0);
     c->dSegPartial = devAlloc<double>(c, std::max(1, c->nseg) * 5);
     // rings and per-step tables

@@ -1301,63 +1301,107 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
 // ---------------------------------------------------------------- per-step kernels ----
 
 struct SourceArgs {
-    const int64_t* nodeOffsets;  // [nsrc+1] into nodeIdx
-    const int64_t* nodeIdx;      // flat element-node indices (canonical)
-    const double* params;        // [nsrc][5]: kind, amp, freq, phase, duration
-    const double* tTable;        // [slots] step times
-    const double* vTable;        // [slots][nsrc] table values (kind 1)
-    const uint8_t* vFlag;        // [slots][nsrc] 1 = overwrite (kind 1)
+    const int64_t* nodes;     // [nnodes] distinct element-node indices covered by a source (ascending)
+    const int32_t* lastSrc;   // [nnodes] the last source (definition order) covering the node
+    const int64_t* coverOff;  // [nnodes+1] into coverSrc
+    const int32_t* coverSrc;  // the sources covering each node, in definition order
+    int64_t nnodes;
+    const double* params;     // [nsrc][5]: kind, amp, freq, phase, duration
+    const double* tTable;     // [slots] step times
+    const double* vTable;     // [slots][nsrc] table values (kind 1)
+    const uint8_t* vFlag;     // [slots][nsrc] 1 = overwrite (kind 1)
     int nsrc;
     int nf;
-    double ic2;                  // 1/c0^2 (5-field variant: rho follows p)
+    double ic2;               // 1/c0^2 (5-field variant: rho follows p)
 };
 
-// Hard-source overwrite of the pressure at the source nodes for step `slot`
-// (src/solver.cpp:455-483): monopole amp*sin(2 pi f t + phase) while t < duration; table
-// sources (formula / file, evaluated on the host) when flagged. Executed by one block.
+// Value of source `src` at step `slot` (time t) and whether it overwrites (src/solver.cpp:455-483):
+// monopole amp*sin(2 pi f t + phase) while t < duration; table sources (formula / file,
+// evaluated on the host) when flagged.
+__device__ inline bool sourceValue(const SourceArgs& s, int src, int64_t slot, double t, double& value) {
+    const double* pr = s.params + src * 5;
+    if (pr[0] == 0.0) {
+        value = pr[1] * sin(2.0 * M_PI * pr[2] * t + pr[3]);
+        return t < pr[4];
+    }
+    value = s.vTable[slot * s.nsrc + src];
+    return s.vFlag[slot * s.nsrc + src] != 0;
+}
+
+// Hard-source overwrite of the pressure (and, with 5 fields, of rho = p/c0^2) at one node.
 template <typename T, bool COMP>
-__device__ inline void applySources(const SourceArgs& s, int64_t slot, T* __restrict__ u, T* __restrict__ ulo) {
-    const double t = s.tTable[slot];
-    for (int src = 0; src < s.nsrc; ++src) {
-        const double* pr = s.params + src * 5;
-        double value;
-        bool active;
-        if (pr[0] == 0.0) {
-            active = t < pr[4];
-            value = pr[1] * sin(2.0 * M_PI * pr[2] * t + pr[3]);
-        } else {
-            active = s.vFlag[slot * s.nsrc + src] != 0;
-            value = s.vTable[slot * s.nsrc + src];
-        }
-        if (!active) continue;
-        const int64_t b = s.nodeOffsets[src], e = s.nodeOffsets[src + 1];
-        const T hi = static_cast<T>(value);
-        const T lo = static_cast<T>(value - static_cast<double>(hi));
+__device__ inline void writeSource(const SourceArgs& s, int64_t node, double value, T* __restrict__ u,
+                                   T* __restrict__ ulo) {
+    const int64_t base = node * s.nf;
+    const T hi = static_cast<T>(value);
+    u[base] = hi;
+    if constexpr (COMP) ulo[base] = static_cast<T>(value - static_cast<double>(hi));
+    if (s.nf == 5) {
         const double rv = value * s.ic2;
         const T rh = static_cast<T>(rv);
-        const T rl = static_cast<T>(rv - static_cast<double>(rh));
-        // node indices of kU entries per thread are loaded (from clamped addresses) before any
-        // store, so one round trip covers kU * blockDim entries
-        constexpr int kU = 8;
-        for (int64_t x0 = b + threadIdx.x; x0 < e; x0 += kU * blockDim.x) {
-            int64_t idx[kU];
+        u[base + 4] = rh;
+        if constexpr (COMP) ulo[base + 4] = static_cast<T>(rv - static_cast<double>(rh));
+    }
+}
+
+// A thread's share of one round of the source table: nodes x0 + q * blockDim.x, q < kSrcU,
+// loaded from clamped addresses before any use (the caller guarantees nnodes > 0).
+constexpr int kSrcU = 2;
+struct SourceRound {
+    int64_t node[kSrcU];
+    int32_t src[kSrcU];
+    int64_t b[kSrcU];
+    int64_t n[kSrcU];  // number of covering sources; 0 for a slot past the end
+};
+
+__device__ inline void loadSourceRound(const SourceArgs& s, int64_t x0, SourceRound& r) {
 #pragma unroll
-            for (int q = 0; q < kU; ++q) {
-                const int64_t x = x0 + static_cast<int64_t>(q) * blockDim.x;
-                idx[q] = s.nodeIdx[x < e ? x : e - 1];
-            }
+    for (int q = 0; q < kSrcU; ++q) {
+        const int64_t x = x0 + static_cast<int64_t>(q) * blockDim.x;
+        const bool ok = x < s.nnodes;
+        const int64_t xc = ok ? x : 0;
+        r.node[q] = s.nodes[xc];
+        r.src[q] = s.lastSrc[xc];
+        const int64_t b = s.coverOff[xc], e = s.coverOff[xc + 1];
+        r.b[q] = b;
+        r.n[q] = ok ? e - b : 0;
+    }
+}
+
+// Every node takes the value of the last source covering it that is active at this step: the
+// reference applies the sources in definition order, so a later active source overwrites an
+// earlier one; a node whose sources are all inactive keeps its state.
+__device__ inline void evalSourceRound(const SourceArgs& s, int64_t slot, double t, const SourceRound& r,
+                                       bool (&act)[kSrcU], double (&v)[kSrcU]) {
 #pragma unroll
-            for (int q = 0; q < kU; ++q) {
-                if (x0 + static_cast<int64_t>(q) * blockDim.x >= e) break;
-                const int64_t base = idx[q] * s.nf;
-                u[base] = hi;
-                if constexpr (COMP) ulo[base] = lo;
-                if (s.nf == 5) {
-                    u[base + 4] = rh;
-                    if constexpr (COMP) ulo[base + 4] = rl;
-                }
-            }
-        }
+    for (int q = 0; q < kSrcU; ++q) {
+        v[q] = 0.0;
+        act[q] = r.n[q] > 0 && sourceValue(s, r.src[q], slot, t, v[q]);
+        for (int64_t k = r.b[q] + r.n[q] - 2; !act[q] && k >= r.b[q]; --k)
+            act[q] = sourceValue(s, s.coverSrc[k], slot, t, v[q]);
+    }
+}
+
+template <typename T, bool COMP>
+__device__ inline void writeSourceRound(const SourceArgs& s, const SourceRound& r, const bool (&act)[kSrcU],
+                                        const double (&v)[kSrcU], T* __restrict__ u, T* __restrict__ ulo) {
+#pragma unroll
+    for (int q = 0; q < kSrcU; ++q)
+        if (act[q]) writeSource<T, COMP>(s, r.node[q], v[q], u, ulo);
+}
+
+// Source overwrite of step `slot` by one block.
+template <typename T, bool COMP>
+__device__ inline void applySources(const SourceArgs& s, int64_t slot, T* __restrict__ u, T* __restrict__ ulo) {
+    if (s.nnodes == 0) return;
+    const double t = s.tTable[slot];
+    for (int64_t x0 = threadIdx.x; x0 < s.nnodes; x0 += static_cast<int64_t>(kSrcU) * blockDim.x) {
+        SourceRound r;
+        bool act[kSrcU];
+        double v[kSrcU];
+        loadSourceRound(s, x0, r);
+        evalSourceRound(s, slot, t, r, act, v);
+        writeSourceRound<T, COMP>(s, r, act, v, u, ulo);
     }
 }
 
@@ -1372,11 +1416,8 @@ struct PostArgs {
     const double* resPartial;   // [nblocks][5] from the final stage
     int64_t nblocks;
     double invN;                // 1/(K*Np)
-    const int64_t* obsOffsets;  // [nobs+1]
-    const int64_t* obsIdx;      // element-node indices
-    const double* obsW;         // IDW weights
-    const int64_t* segBegin;    // [nseg][2] begin/end of observer node-list segments (<= 256 entries)
-    const int32_t* segObs;      // [nseg] observer of the segment
+    const int64_t* obsIdx;      // [max(nseg,1)][256] element-node indices; observer node lists in
+    const double* obsW;         //   segments of 256 entries, padded with weight 0 (IDW weights)
     const int32_t* obsSeg0;     // [nobs+1] first segment of each observer
     double* segPartial;         // [nseg][5]
     int nseg;
@@ -1392,64 +1433,101 @@ struct PostArgs {
 
 // End-of-step work in one block of kPostBlock threads: residuals (src/solver.cpp:518-547;
 // thread t sums the stage blocks t, t + kPostBlock, ... in that order, then a fixed-order tree),
-// inverse-distance-weighted observers (src/solver.cpp:552-573; one wave per node-list segment
-// of <= 256 entries), then the source overwrite of the next step of the batch and the step
-// counter advance. Loads of a phase are issued together from clamped addresses.
+// inverse-distance-weighted observers (src/solver.cpp:552-573; one wave per segment of 256
+// node-list entries), then the source overwrite of the next step of the batch and the step
+// counter advance. The loads that depend on no other load (counter, the first observer
+// segment, the source table, observer ranges, residual partials) are issued first, then the
+// ones that depend on them (state at observer nodes, step time and source parameters): with up
+// to 4096 stage blocks and one observer segment per wave a step costs three memory round trips.
 constexpr int kPostBlock = 1024;
 
 template <typename T, bool COMP>
 __global__ __launch_bounds__(kPostBlock) void postKernel(PostArgs p, SourceArgs s, T* __restrict__ u,
                                                          T* __restrict__ ulo) {
     constexpr int NW = kPostBlock / 64;
+    constexpr int U = 4;
     __shared__ double red[NW][5];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    // ---- round trip 1 ----
     const int64_t slot = *p.stepCounter;
-    // residual partial sums of this thread, 8 stage blocks per round trip
-    double acc[5] = {0, 0, 0, 0, 0};
-    if (p.computeResiduals) {
-        constexpr int U = 8;
-        for (int64_t b0 = tid; b0 < p.nblocks; b0 += U * kPostBlock) {
-            double v[U][5];
-#pragma unroll
-            for (int q = 0; q < U; ++q) {
-                const int64_t b = b0 + static_cast<int64_t>(q) * kPostBlock;
-                const int64_t bc = b < p.nblocks ? b : p.nblocks - 1;
-#pragma unroll
-                for (int k = 0; k < 5; ++k) v[q][k] = p.resPartial[bc * 5 + k];
-            }
-#pragma unroll
-            for (int q = 0; q < U; ++q)
-                if (b0 + static_cast<int64_t>(q) * kPostBlock < p.nblocks)
-#pragma unroll
-                    for (int k = 0; k < 5; ++k) acc[k] += v[q][k];
-        }
-    }
-    // observer segments: one wave per segment, 4 entries per lane
-    for (int sg = wave; sg < p.nseg; sg += NW) {
-        const int64_t x0 = p.segBegin[2 * sg], x1 = p.segBegin[2 * sg + 1];
-        int64_t base[4];
-        double w[4];
+    const int64_t nsteps = *p.batchSteps;
+    int64_t oIdx[4];
+    double oW[4];
+    {
+        const int64_t x0 = static_cast<int64_t>(wave < p.nseg ? wave : 0) * 256;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int64_t x = x0 + lane + 64 * q;
-            const int64_t xc = x < x1 ? x : x0;
-            base[q] = p.obsIdx[xc] * p.nf;
-            w[q] = x < x1 ? p.obsW[xc] : 0.0;
+            oIdx[q] = p.obsIdx[x0 + lane + 64 * q];
+            oW[q] = p.obsW[x0 + lane + 64 * q];
         }
-        double vq[4][4];
+    }
+    const int oc = tid < p.nobs ? tid : 0;
+    const int sg0 = p.obsSeg0[oc], sg1 = p.obsSeg0[oc + 1];
+    SourceRound sr;
+    const bool haveSrc = s.nnodes > 0;
+    if (haveSrc) loadSourceRound(s, tid, sr);
+    // residual partial sums of this thread: stage blocks tid, tid + kPostBlock, ... in order,
+    // U per round trip
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int64_t b0 = tid; b0 < p.nblocks; b0 += static_cast<int64_t>(U) * kPostBlock) {
+        double rv[U][5];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < U; ++q) {
+            const int64_t b = b0 + static_cast<int64_t>(q) * kPostBlock;
+            const int64_t bc = b < p.nblocks ? b : p.nblocks - 1;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                vq[q][k] = static_cast<double>(u[base[q] + k]);
-                if constexpr (COMP) vq[q][k] += static_cast<double>(ulo[base[q] + k]);
+            for (int k = 0; k < 5; ++k) rv[q][k] = p.resPartial[bc * 5 + k];
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q)
+            if (b0 + static_cast<int64_t>(q) * kPostBlock < p.nblocks)
+#pragma unroll
+                for (int k = 0; k < 5; ++k) acc[k] += rv[q][k];
+    }
+    if (!p.computeResiduals)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) acc[k] = 0.0;
+
+    // ---- round trip 2 ----
+    const bool srcStep = slot + 1 < nsteps;
+    const int64_t slotNext = srcStep ? slot + 1 : slot;
+    const double tNext = s.tTable[slotNext];
+    double vq[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            vq[q][k] = static_cast<double>(u[oIdx[q] * p.nf + k]);
+            if constexpr (COMP) vq[q][k] += static_cast<double>(ulo[oIdx[q] * p.nf + k]);
+        }
+    bool srcAct[kSrcU] = {};
+    double srcVal[kSrcU] = {};
+    if (haveSrc) evalSourceRound(s, slotNext, tNext, sr, srcAct, srcVal);
+
+    // observer segments: one wave per segment, 4 entries per lane
+    for (int sg = wave; sg < p.nseg; sg += NW) {
+        if (sg != wave) {
+            const int64_t x0 = static_cast<int64_t>(sg) * 256;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                oIdx[q] = p.obsIdx[x0 + lane + 64 * q];
+                oW[q] = p.obsW[x0 + lane + 64 * q];
             }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    vq[q][k] = static_cast<double>(u[oIdx[q] * p.nf + k]);
+                    if constexpr (COMP) vq[q][k] += static_cast<double>(ulo[oIdx[q] * p.nf + k]);
+                }
+        }
         double ob[5] = {0, 0, 0, 0, 0};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) ob[k] += vq[q][k] * w[q];
-            ob[4] += w[q];
+            for (int k = 0; k < 4; ++k) ob[k] += vq[q][k] * oW[q];
+            ob[4] += oW[q];
         }
 #pragma unroll
         for (int k = 0; k < 5; ++k)
@@ -1476,11 +1554,21 @@ __global__ __launch_bounds__(kPostBlock) void postKernel(PostArgs p, SourceArgs 
     // observers: sum of the observer's segment partials in segment order
     if (tid < p.nobs) {
         double ob[5] = {0, 0, 0, 0, 0};
-        for (int sg = p.obsSeg0[tid]; sg < p.obsSeg0[tid + 1]; ++sg)
+        for (int sg = sg0; sg < sg1; ++sg)
             for (int k = 0; k < 5; ++k) ob[k] += p.segPartial[sg * 5 + k];
         for (int k = 0; k < 4; ++k) p.obsRing[(slot * p.nobs + tid) * 4 + k] = ob[k] / ob[4];
     }
-    if (slot + 1 < *p.batchSteps) applySources<T, COMP>(s, slot + 1, u, ulo);
+    // source overwrite of the next step of the batch
+    if (srcStep && haveSrc) {
+        writeSourceRound<T, COMP>(s, sr, srcAct, srcVal, u, ulo);
+        for (int64_t x0 = tid + static_cast<int64_t>(kSrcU) * kPostBlock; x0 < s.nnodes;
+             x0 += static_cast<int64_t>(kSrcU) * kPostBlock) {
+            SourceRound r;
+            loadSourceRound(s, x0, r);
+            evalSourceRound(s, slotNext, tNext, r, srcAct, srcVal);
+            writeSourceRound<T, COMP>(s, r, srcAct, srcVal, u, ulo);
+        }
+    }
     __syncthreads();
     if (tid == 0) *p.stepCounter = slot + 1;
 }

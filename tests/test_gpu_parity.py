@@ -99,6 +99,42 @@ def test_rk4_3d_absorbing_monopole_matches_oracle(kernel, monkeypatch):
     np.testing.assert_allclose(r.observers, ref["observers"], rtol=1e-8, atol=1e-14)
 
 
+def test_overlapping_sources_match_oracle():
+    """Sources applied in definition order (src/solver.cpp:455-483): where sources overlap, the
+    last active one sets the node. Three overlapping monopoles whose durations end at different
+    steps and a table source that is active on every third step only, 60 RK4 steps in batches of
+    7 (sources of a batch's first step and of later steps are written by different kernels)."""
+    print("case overlapping sources 3d n4 p2", flush=True)
+    groups = [(ALL3, 0)]
+    v0 = (30.0, 0.0, 0.0)
+    m, bc, gn = make_case(3, 4, 2, groups, jitter=0.15, seed=6)
+    o = oracle_for(m, gn, groups, v0, RHO0, 340.0)
+    dt, steps = 2e-5, 60
+    mono = [(0.0, 0.0, 0.0, 0.45, 0.1, 200.0, 0.0, 1.0), (0.1, 0.0, 0.0, 0.35, 0.2, 350.0, 0.5, 30.5 * dt),
+            (-0.05, 0.05, 0.0, 0.25, 0.05, 500.0, 1.0, 10.5 * dt)]
+
+    def table(t):
+        k = int(round(t / dt))
+        return 0.03 * np.cos(900.0 * t) if k % 3 == 0 else None
+
+    src = [dict(kind=0, x=x, y=y, z=z, size=r, amp=a, freq=f, phase=ph, duration=d) for x, y, z, r, a, f, ph, d in mono]
+    src.append(dict(kind=1, x=0.05, y=0.0, z=0.05, size=0.3))
+    u0 = gaussian_state(m, size=0.05)
+    ref = o.run(u0, 0.0, (steps - 0.5) * dt, dt, 25 * dt, sources=src, table_fn=lambda i, t: table(t),
+                observers=[(0.0, 0.0, 0.0, 0.3)])
+    s = dg.Solver(m.node_coords, m.element_vertices, 2, bc, v0=v0, rho0=RHO0, c0=340.0, steps_per_graph=7)
+    s.set_state(u0)
+    counts = [s.add_monopole(*mp) for mp in mono] + [s.add_table_source(0.05, 0.0, 0.05, 0.3, table)]
+    assert min(counts) > 0
+    s.add_observer(0.0, 0.0, 0.0, 0.3)
+    r = s.run(0.0, (steps - 0.5) * dt, dt, 25 * dt)
+    u = s.get_state()
+    for f in range(4):
+        assert rel_l2(u[f], ref["u"][f]) < 1e-10, (f, rel_l2(u[f], ref["u"][f]))
+    np.testing.assert_allclose(r.residuals, ref["residuals"], rtol=1e-8, atol=1e-300)
+    np.testing.assert_allclose(r.observers, ref["observers"], rtol=1e-8, atol=1e-14)
+
+
 def test_euler_matches_oracle():
     print("case euler 2d", flush=True)
     groups = [(0x3, 1), (0xC, 0)]
