@@ -8,6 +8,10 @@
            mapped to reflecting walls; forward Euler (Euler1), dt = 5e-6 to t = 5e-4
            (100 steps), v0 = (20, -10, 0), Gaussian initial pressure, a dipole source (two
            monopoles in antiphase), one observer; snapshot at step 50.
+  square2_tables: the same mesh, absorbing, RK4 over 100 steps with a quadrupole, a formula
+           source overlapping it (later in key order, so it wins while active; its duration
+           ends at step 60) and a file source (data/data.csv, linear interpolation of the
+           float-rounded table, include/configParser.h:38-58), two observers.
 
 The CLI prints residuals and observer values with 6 significant digits, so those are held to
 rtol 1e-5; VTU cell averages (17 digits) to 1e-10 of the field's maximum.
@@ -61,7 +65,44 @@ CASE_DEFS = {
                           "source1 = dipole, -2,0,0, 0.6, 0.3, 2000, 0, 1\n"
                           "observer1 = 3,0,0,0.5\n",
                     ic=(1.0, 1.0, 0.0, 2.0, 1.0), observers=[(3.0, 0.0, 0.0, 0.5)]),
+    "square2_tables": dict(mesh="doc/2d/square2.msh", files=["data/data.csv"], t_end=5e-4, dt=5e-6, rate=2.5e-4,
+                           method="Runge-Kutta", bc="Abs = Absorbing", v0=(20.0, -10.0, 0.0), integrator=0, bc_type=0,
+                           extra="initialCondtition1 = gaussian, 1,1,0, 2, 1\n"
+                                 "source1 = quadrupole, -2,0,0, 0.8, 0.4, 2500, 0, 1\n"
+                                 "source2 = formula, \"0.2*sin(2*pi*3000*t)+0.05\", -2,0,0, 0.5, 3.0e-4\n"
+                                 "source3 = file,\"data/data.csv\", 1.5,-1,0, 0.5\n"
+                                 "observer1 = 3,0,0,0.5\nobserver2 = -1.5,0.5,0,0.6\n",
+                           ic=(1.0, 1.0, 0.0, 2.0, 1.0), observers=[(3.0, 0.0, 0.0, 0.5), (-1.5, 0.5, 0.0, 0.6)]),
 }
+
+
+def _interpolate(rows, t):
+    """Sources::interpolate_value (include/configParser.h:38-58): linear between the bracketing
+    rows found by bisection, 0 outside [t_first, t_last]."""
+    if len(rows) < 2 or not rows[0, 0] <= t <= rows[-1, 0]:
+        return 0.0
+    lo = min(int(np.searchsorted(rows[:, 0], t, side="right")) - 1, len(rows) - 2)
+    hi = lo + 1
+    return rows[lo, 1] + ((rows[hi, 1] - rows[lo, 1]) / (rows[hi, 0] - rows[lo, 0])) * (t - rows[lo, 0])
+
+
+def _oracle_sources(c, summary):
+    """Oracle source list (kind 0 monopoles; kind 1 for formula and file sources, whose values
+    come from the table function) and the table function."""
+    src, tables = [], {}
+    for i, x in enumerate(summary["sources"]):
+        if x["kind"] == 0:
+            src.append(x)
+            continue
+        src.append(dict(kind=1, x=x["x"], y=x["y"], z=x["z"], size=x["size"]))
+        if x["kind"] == 1:
+            tables[i] = (lambda t, f=x["formula"], d=x["duration"]: dg.eval_formula(f, t) if t < d else None)
+        else:
+            (path,) = c["files"]
+            rows = dg.read_source_table(os.path.join(CASES, path))
+            assert len(rows) == x["rows"]
+            tables[i] = (lambda t, r=rows: _interpolate(r, t))
+    return src, (lambda i, t: tables[i](t))
 
 
 def _read_residuals(path):
@@ -78,8 +119,9 @@ def case(request, tmp_path_factory):
     if not os.path.exists(EXE):
         pytest.fail("dgalerkin is not built (make -C dgfem-acoustic_amd)")
     d = tmp_path_factory.mktemp(name)
-    os.makedirs(d / os.path.dirname(c["mesh"]))
-    shutil.copyfile(os.path.join(CASES, c["mesh"]), d / c["mesh"])
+    for rel in [c["mesh"]] + c.get("files", []):
+        os.makedirs(d / os.path.dirname(rel), exist_ok=True)
+        shutil.copyfile(os.path.join(CASES, rel), d / rel)
     (d / "case.conf").write_text(CONF.format(rho0=RHO0, c0=C0, **c))
     cwd = os.getcwd()
     try:
@@ -101,9 +143,8 @@ def case(request, tmp_path_factory):
     x0, y0, z0, size, amp = c["ic"]
     u0 = np.zeros((4, len(xs)))
     u0[0] = amp * np.exp(-((xs[:, 0] - x0) ** 2 + (xs[:, 1] - y0) ** 2 + (xs[:, 2] - z0) ** 2) / size)
-    src = summary["sources"]
-    assert all(s["kind"] == 0 for s in src)
-    run = dict(integrator=c["integrator"], sources=src, observers=c["observers"])
+    src, table_fn = _oracle_sources(c, summary)
+    run = dict(integrator=c["integrator"], sources=src, table_fn=table_fn, observers=c["observers"])
     full = o.run(u0, 0.0, c["t_end"], c["dt"], c["rate"], **run)
     assert np.isfinite(full["observers"]).all()
     half = o.run(u0, 0.0, c["t_end"], c["dt"], c["rate"], max_steps=int(full["saves"][1]), **run)
@@ -117,6 +158,10 @@ def test_case_shape(case):
         assert s["order"] == 2 and s["bc_counts"] == {"absorbing": 0, "reflecting": 160}
         assert [x["pole"] for x in s["sources"]] == [1, 1]
         assert [x["phase"] for x in s["sources"]] == [0.0, np.pi]
+    elif case["name"] == "square2_tables":
+        assert s["order"] == 2 and s["bc_counts"] == {"absorbing": 160, "reflecting": 0}
+        assert [x["kind"] for x in s["sources"]] == [0, 0, 0, 0, 1, 2]
+        assert [x["pole"] for x in s["sources"]][:4] == [2, 2, 2, 2]
     else:
         assert s["order"] == 1 and s["bc_counts"] == {"absorbing": 3198, "reflecting": 0}
     assert case["full"]["steps"] == 100
