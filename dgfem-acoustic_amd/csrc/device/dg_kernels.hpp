@@ -1552,11 +1552,12 @@ __global__ __launch_bounds__(kPostBlock) void postKernel(PostArgs p, SourceArgs 
         if (!isfinite(v)) atomicCAS(p.nonfinite, -1, static_cast<int>(slot));
     }
     // observers: sum of the observer's segment partials in segment order
-    if (tid < p.nobs) {
+    for (int o = tid; o < p.nobs; o += kPostBlock) {
+        const int s0 = o == tid ? sg0 : p.obsSeg0[o], s1 = o == tid ? sg1 : p.obsSeg0[o + 1];
         double ob[5] = {0, 0, 0, 0, 0};
-        for (int sg = sg0; sg < sg1; ++sg)
+        for (int sg = s0; sg < s1; ++sg)
             for (int k = 0; k < 5; ++k) ob[k] += p.segPartial[sg * 5 + k];
-        for (int k = 0; k < 4; ++k) p.obsRing[(slot * p.nobs + tid) * 4 + k] = ob[k] / ob[4];
+        for (int k = 0; k < 4; ++k) p.obsRing[(slot * p.nobs + o) * 4 + k] = ob[k] / ob[4];
     }
     // source overwrite of the next step of the batch
     if (srcStep && haveSrc) {

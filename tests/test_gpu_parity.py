@@ -135,6 +135,29 @@ def test_overlapping_sources_match_oracle():
     np.testing.assert_allclose(r.observers, ref["observers"], rtol=1e-8, atol=1e-14)
 
 
+def test_more_observers_than_post_threads():
+    """1,100 observers (more than the 1,024 threads of the end-of-step kernel), some of them
+    sharing nodes, 12 RK4 steps against the oracle."""
+    print("case 1100 observers 3d n3 p2", flush=True)
+    groups = [(ALL3, 0)]
+    v0 = (30.0, 0.0, 0.0)
+    m, bc, gn = make_case(3, 3, 2, groups, jitter=0.15, seed=8)
+    o = oracle_for(m, gn, groups, v0, RHO0, 340.0)
+    rng = np.random.default_rng(3)
+    obs = [(float(x), float(y), float(z), float(r)) for (x, y, z), r in
+           zip(rng.uniform(-0.8, 0.8, (1100, 3)), rng.uniform(0.25, 0.5, 1100))]
+    dt, steps = 2e-5, 12
+    u0 = gaussian_state(m, size=0.05)
+    ref = o.run(u0, 0.0, (steps - 0.5) * dt, dt, 1e9, observers=obs)
+    s = dg.Solver(m.node_coords, m.element_vertices, 2, bc, v0=v0, rho0=RHO0, c0=340.0, steps_per_graph=5)
+    s.set_state(u0)
+    counts = [s.add_observer(*ob) for ob in obs]
+    assert min(counts) > 0
+    r = s.run(0.0, (steps - 0.5) * dt, dt, 1e9)
+    assert r.observers.shape == (steps, 1100, 4)
+    np.testing.assert_allclose(r.observers, ref["observers"], rtol=1e-8, atol=1e-14)
+
+
 def test_euler_matches_oracle():
     print("case euler 2d", flush=True)
     groups = [(0x3, 1), (0xC, 0)]
