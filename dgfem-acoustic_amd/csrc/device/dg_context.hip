@@ -139,9 +139,10 @@ dgk::Physics<T> makePhysics(const dg_physics& ph) {
     return o;
 }
 
-template <int DIM, int P, int NF, typename T, bool COMP>
-void launchStage(dg_context* c, int stage, const void* qin, void* qout, void* uout, bool snap) {
-    using dgk::stageKernel;
+// Arguments of one stage launch: stage input qin, stage output qout (next stage input), final
+// state uout, residual snapshot on or off.
+template <typename T, bool COMP>
+dgk::StageArgs<T> stageArgs(dg_context* c, int stage, const void* qin, void* qout, void* uout, bool snap) {
     dgk::StageArgs<T> a{};
     a.qin = static_cast<const T*>(qin);
     a.qout = static_cast<T*>(qout);
@@ -162,6 +163,13 @@ void launchStage(dg_context* c, int stage, const void* qin, void* qout, void* uo
     a.dt = static_cast<T>(c->curDt);
     a.stage = stage;
     a.ph = makePhysics<T>(c->phys);
+    return a;
+}
+
+template <int DIM, int P, int NF, typename T, bool COMP>
+void launchStage(dg_context* c, int stage, const void* qin, void* qout, void* uout, bool snap) {
+    using dgk::stageKernel;
+    const dgk::StageArgs<T> a = stageArgs<T, COMP>(c, stage, qin, qout, uout, snap);
     const dim3 grid(static_cast<unsigned>(c->nblocks)), block(dgk::StageWaves<DIM, P, NF, T>::kBlockSize);
     switch (dgk::stageMode(stage)) {
         case dgk::kModeFirst:
@@ -203,26 +211,7 @@ dgk::SourceArgs sourceArgs(dg_context* c) {
 // previous step's post kernel.
 template <int DIM, int P, typename T, bool COMP>
 void launchStageWave(dg_context* c, int stage, const void* qin, void* qout, void* uout, bool snap) {
-    dgk::StageArgs<T> a{};
-    a.qin = static_cast<const T*>(qin);
-    a.qout = static_cast<T*>(qout);
-    a.u = static_cast<T*>(c->u);
-    a.uout = static_cast<T*>(uout);
-    a.ulo = COMP ? static_cast<T*>(c->ulo) : nullptr;
-    a.acc = static_cast<T*>(c->acc);
-    a.geo = static_cast<const T*>(c->geo);
-    a.fgeo = static_cast<const T*>(c->fgeo);
-    a.nbr = c->nbr;
-    a.ghost = static_cast<const T*>(c->ghost);
-    a.code = c->code;
-    a.ops = static_cast<const T*>(c->ops);
-    a.iops = c->iops;
-    a.snap = snap ? static_cast<const T*>(c->snap) : nullptr;
-    a.resPartial = c->resPartial;
-    a.K = c->K;
-    a.dt = static_cast<T>(c->curDt);
-    a.stage = stage;
-    a.ph = makePhysics<T>(c->phys);
+    const dgk::StageArgs<T> a = stageArgs<T, COMP>(c, stage, qin, qout, uout, snap);
     hipLaunchKernelGGL((dgk::stageKernelWave<DIM, P, T, COMP>), dim3(static_cast<unsigned>(c->nblocks)),
                        dim3(dgk::kBlock), 0, c->stream, a, static_cast<const T*>(c->afrag));
 }
@@ -230,26 +219,7 @@ void launchStageWave(dg_context* c, int stage, const void* qin, void* qout, void
 template <int DIM, int P, typename T, bool COMP>
 void launchStageMM(dg_context* c, int stage, const void* qin, void* qout, void* uout, bool snap) {
     using dgk::stageKernelMM;
-    dgk::StageArgs<T> a{};
-    a.qin = static_cast<const T*>(qin);
-    a.qout = static_cast<T*>(qout);
-    a.u = static_cast<T*>(c->u);
-    a.uout = static_cast<T*>(uout);
-    a.ulo = COMP ? static_cast<T*>(c->ulo) : nullptr;
-    a.acc = static_cast<T*>(c->acc);
-    a.geo = static_cast<const T*>(c->geo);
-    a.fgeo = static_cast<const T*>(c->fgeo);
-    a.nbr = c->nbr;
-    a.ghost = static_cast<const T*>(c->ghost);
-    a.code = c->code;
-    a.ops = static_cast<const T*>(c->ops);
-    a.iops = c->iops;
-    a.snap = snap ? static_cast<const T*>(c->snap) : nullptr;
-    a.resPartial = c->resPartial;
-    a.K = c->K;
-    a.dt = static_cast<T>(c->curDt);
-    a.stage = stage;
-    a.ph = makePhysics<T>(c->phys);
+    const dgk::StageArgs<T> a = stageArgs<T, COMP>(c, stage, qin, qout, uout, snap);
     const T* af = static_cast<const T*>(c->afrag);
     const dim3 grid(static_cast<unsigned>(c->nblocks)), block(dgk::kBlock);
     switch (dgk::stageMode(stage)) {
