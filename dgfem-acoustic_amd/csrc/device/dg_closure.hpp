@@ -175,11 +175,95 @@ struct ClosureArgs {
     dgk::Physics<T> ph;
 };
 
-// One wave per face, 16 faces per block: the network is staged in LDS once per 16 faces.
-constexpr int kClosureFacesPerBlock = 16;
-constexpr int kClosureBlock = 64 * kClosureFacesPerBlock;
+// Forward pass of F samples by one wave (lane = hidden unit): each weight read from LDS serves
+// F samples and the F accumulations are independent. Inputs sx[F][XS] (XS >= NIN, a multiple of
+// 4), activations sa[F][kH], outputs sy[F][NOUT]; sx and sa2 may share storage (sx is consumed
+// before sa2 is written). Per output the sum runs over the inputs in ascending order, as in
+// mlpForwardWave.
+template <int NIN, int NOUT, int F, int XS>
+__device__ inline void mlpForwardBatch(const float* net, const float* sx, float* sa1, float* sa2, float* sy, int lane) {
+    using N = NetLds<NIN, NOUT>;
+    static_assert(XS % 4 == 0 && XS >= NIN && NOUT <= 16 && F % 4 == 0, "batch layout");
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    float z[F];
+    {
+        const float* w = net + N::kW1 + lane * N::S1;
+        const float b = net[N::kB1 + lane];
+#pragma unroll
+        for (int f = 0; f < F; ++f) z[f] = b;
+        constexpr int NIN4 = NIN & ~3;
+#pragma unroll 2
+        for (int i = 0; i < NIN4; i += 4) {
+            const float w0 = w[i], w1 = w[i + 1], w2 = w[i + 2], w3 = w[i + 3];
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                const F4 x = *reinterpret_cast<const F4*>(sx + f * XS + i);
+                z[f] = fmaf(w3, x[3], fmaf(w2, x[2], fmaf(w1, x[1], fmaf(w0, x[0], z[f]))));
+            }
+        }
+#pragma unroll
+        for (int i = NIN4; i < NIN; ++i) {
+            const float wi = w[i];
+#pragma unroll
+            for (int f = 0; f < F; ++f) z[f] = fmaf(wi, sx[f * XS + i], z[f]);
+        }
+#pragma unroll
+        for (int f = 0; f < F; ++f) sa1[f * kH + lane] = tanhf(z[f]);
+    }
+    waveFence();
+    {
+        const float* w = net + N::kW2 + lane * N::S2;
+        const float b = net[N::kB2 + lane];
+#pragma unroll
+        for (int f = 0; f < F; ++f) z[f] = b;
+#pragma unroll 2
+        for (int k = 0; k < kH; k += 4) {
+            const float w0 = w[k], w1 = w[k + 1], w2 = w[k + 2], w3 = w[k + 3];
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                const F4 x = *reinterpret_cast<const F4*>(sa1 + f * kH + k);
+                z[f] = fmaf(w3, x[3], fmaf(w2, x[2], fmaf(w1, x[1], fmaf(w0, x[0], z[f]))));
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < F; ++f) sa2[f * kH + lane] = tanhf(z[f]);
+    }
+    waveFence();
+    {
+        // lane = output j + 16 g: samples g, g + 4, ...
+        const int j = lane & 15, g = lane >> 4;
+        if (j < NOUT) {
+            constexpr int FG = F / 4;
+            const float* w = net + N::kW3 + j * N::S2;
+            const float b = net[N::kB3 + j];
+            float y[FG];
+#pragma unroll
+            for (int c = 0; c < FG; ++c) y[c] = b;
+#pragma unroll 2
+            for (int k = 0; k < kH; k += 4) {
+                const float w0 = w[k], w1 = w[k + 1], w2 = w[k + 2], w3 = w[k + 3];
+#pragma unroll
+                for (int c = 0; c < FG; ++c) {
+                    const F4 x = *reinterpret_cast<const F4*>(sa2 + (g + 4 * c) * kH + k);
+                    y[c] = fmaf(w3, x[3], fmaf(w2, x[2], fmaf(w1, x[1], fmaf(w0, x[0], y[c]))));
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < FG; ++c) sy[(g + 4 * c) * NOUT + j] = y[c];
+        }
+    }
+    waveFence();
+}
 
-// Exterior states of the learned faces for one stage.
+// kClosureFacesPerWave faces per wave, kClosureWaves waves per block: the network is staged
+// in LDS once per block and every weight read serves a wave's faces together.
+constexpr int kClosureFacesPerWave = 8;
+constexpr int kClosureWaves = 4;
+constexpr int kClosureFacesPerBlock = kClosureFacesPerWave * kClosureWaves;
+constexpr int kClosureBlock = 64 * kClosureWaves;
+
+// Exterior states of the learned faces for one stage. Face-node work runs with lane =
+// (face slot, node): 16 lanes per face (NFP <= 16), 4 faces per pass.
 template <int DIM, int P, int NF, typename T>
 __global__ __launch_bounds__(kClosureBlock) void closureKernel(ClosureArgs<T> a) {
     using S = dgk::Sizes<DIM, P>;
@@ -187,106 +271,127 @@ __global__ __launch_bounds__(kClosureBlock) void closureKernel(ClosureArgs<T> a)
     constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC;
     constexpr int NT = DIM - 1;  // tangential directions
     constexpr int NIN = DIM * NFP + 2, NOUT = NFP;
+    constexpr int FW = kClosureFacesPerWave, NPASS = FW / 4;
+    constexpr int XS = (NIN + 3) & ~3;
+    static_assert(NFP <= 16 && XS <= kH, "closure lane layout");
     using N = NetLds<NIN, NOUT>;
     __shared__ float net[N::kTotal];
-    __shared__ float buf[kClosureFacesPerBlock][NIN + 2 * kH + NOUT + 2];
+    __shared__ __align__(16) float bufX[kClosureWaves][FW * kH];   // inputs, then second activations
+    __shared__ __align__(16) float bufA[kClosureWaves][FW * kH];   // first activations
+    __shared__ float bufY[kClosureWaves][FW * NOUT];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int64_t idx = static_cast<int64_t>(blockIdx.x) * kClosureFacesPerBlock + wave;
-    const int64_t ef = a.faces[idx < a.nfaces ? idx : a.nfaces - 1];  // issued with the network loads
+    const int slot = lane >> 4, node = lane & 15;
+    const bool nodeLane = node < NFP;
+    const int64_t f0 = static_cast<int64_t>(blockIdx.x) * kClosureFacesPerBlock + wave * FW;
+    // face records of this lane's faces, issued with the network loads
+    int64_t ef[NPASS];
+#pragma unroll
+    for (int ps = 0; ps < NPASS; ++ps) {
+        const int64_t idx = f0 + ps * 4 + slot;
+        ef[ps] = a.faces[idx < a.nfaces ? idx : a.nfaces - 1];
+    }
     loadNet<NIN, NOUT, kClosureBlock>(a.w, net, tid);
     __syncthreads();
-    if (idx >= a.nfaces) return;  // whole wave leaves; no block barrier follows
-    float* sx = buf[wave];
-    float* sa1 = sx + NIN;
-    float* sa2 = sa1 + kH;
-    float* sy = sa2 + kH;
-    const int64_t e = ef / NFC;
-    const int f = static_cast<int>(ef - e * NFC);
+    float* sx = bufX[wave];
     const T c0 = a.ph.c0;
-    // face frame: outward unit normal n and tangents t[NT]
-    T n[3] = {a.fgeo[(e * NFC + f) * 4], a.fgeo[(e * NFC + f) * 4 + 1], T(0)};
-    T t[NT][3];
-    T vn0, vt0;
-    if constexpr (DIM == 2) {
-        t[0][0] = -n[1];
-        t[0][1] = n[0];
-        t[0][2] = T(0);
-        vn0 = a.ph.v0[0] * n[0] + a.ph.v0[1] * n[1];
-        vt0 = a.ph.v0[0] * t[0][0] + a.ph.v0[1] * t[0][1];
-    } else {
-        n[2] = a.fgeo[(e * NFC + f) * 4 + 2];
-        vn0 = a.ph.v0[0] * n[0] + a.ph.v0[1] * n[1] + a.ph.v0[2] * n[2];
-        // t1 along the tangential mean flow, else along the axis least aligned with n
-        T u[3];
-        for (int c = 0; c < 3; ++c) u[c] = a.ph.v0[c] - vn0 * n[c];
-        vt0 = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-        if (!(vt0 > T(1e-6) * c0)) {
-            const T ax = fabs(n[0]), ay = fabs(n[1]), az = fabs(n[2]);
-            const int k = (ax <= ay && ax <= az) ? 0 : (ay <= az ? 1 : 2);
-            for (int c = 0; c < 3; ++c) u[c] = (c == k ? T(1) : T(0)) - n[k] * n[c];
-            vt0 = T(0);
-        }
-        const T iu = T(1) / sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-        for (int c = 0; c < 3; ++c) t[0][c] = u[c] * iu;
-        t[1][0] = n[1] * t[0][2] - n[2] * t[0][1];
-        t[1][1] = n[2] * t[0][0] - n[0] * t[0][2];
-        t[1][2] = n[0] * t[0][1] - n[1] * t[0][0];
-    }
-    T qm[NF], wo = T(0), wi = T(0), tt[NT];
-    for (int c = 0; c < NT; ++c) tt[c] = T(0);
-    if (lane < NFP) {
-        const int node = a.iops[L::kMask + f * NFP + lane];
-        const T* q = a.qin + (e * NP + node) * NF;
+    // per pass: this lane's face frame and node characteristics
+    T n[NPASS][3], t[NPASS][NT][3], vn0[NPASS], wo[NPASS], wi[NPASS], tt[NPASS][NT], qm[NPASS][NF];
+    float sc[NPASS];
 #pragma unroll
-        for (int k = 0; k < NF; ++k) qm[k] = q[k];
-        const T Pv = qm[0] * a.ph.irc0;
-        T Nn = qm[1] * n[0] + qm[2] * n[1];
-        if constexpr (DIM == 3) Nn += qm[3] * n[2];
-        wo = Pv + Nn;
-        wi = Pv - Nn;
+    for (int ps = 0; ps < NPASS; ++ps) {
+        const int64_t e = ef[ps] / NFC;
+        const int f = static_cast<int>(ef[ps] - e * NFC);
+        n[ps][0] = a.fgeo[(e * NFC + f) * 4];
+        n[ps][1] = a.fgeo[(e * NFC + f) * 4 + 1];
+        n[ps][2] = DIM == 3 ? a.fgeo[(e * NFC + f) * 4 + 2] : T(0);
+        const int nd = a.iops[L::kMask + f * NFP + (nodeLane ? node : 0)];
+        const T* q = a.qin + (e * NP + nd) * NF;
+#pragma unroll
+        for (int k = 0; k < NF; ++k) qm[ps][k] = q[k];
+    }
+#pragma unroll
+    for (int ps = 0; ps < NPASS; ++ps) {
+        T vt0;
+        if constexpr (DIM == 2) {
+            t[ps][0][0] = -n[ps][1];
+            t[ps][0][1] = n[ps][0];
+            t[ps][0][2] = T(0);
+            vn0[ps] = a.ph.v0[0] * n[ps][0] + a.ph.v0[1] * n[ps][1];
+            vt0 = a.ph.v0[0] * t[ps][0][0] + a.ph.v0[1] * t[ps][0][1];
+        } else {
+            vn0[ps] = a.ph.v0[0] * n[ps][0] + a.ph.v0[1] * n[ps][1] + a.ph.v0[2] * n[ps][2];
+            // t1 along the tangential mean flow, else along the axis least aligned with n
+            T u[3];
+            for (int c = 0; c < 3; ++c) u[c] = a.ph.v0[c] - vn0[ps] * n[ps][c];
+            vt0 = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            if (!(vt0 > T(1e-6) * c0)) {
+                const T ax = fabs(n[ps][0]), ay = fabs(n[ps][1]), az = fabs(n[ps][2]);
+                const int k = (ax <= ay && ax <= az) ? 0 : (ay <= az ? 1 : 2);
+                for (int c = 0; c < 3; ++c) u[c] = (c == k ? T(1) : T(0)) - n[ps][k] * n[ps][c];
+                vt0 = T(0);
+            }
+            const T iu = T(1) / sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            for (int c = 0; c < 3; ++c) t[ps][0][c] = u[c] * iu;
+            t[ps][1][0] = n[ps][1] * t[ps][0][2] - n[ps][2] * t[ps][0][1];
+            t[ps][1][1] = n[ps][2] * t[ps][0][0] - n[ps][0] * t[ps][0][2];
+            t[ps][1][2] = n[ps][0] * t[ps][0][1] - n[ps][1] * t[ps][0][0];
+        }
+        const T Pv = qm[ps][0] * a.ph.irc0;
+        T Nn = qm[ps][1] * n[ps][0] + qm[ps][2] * n[ps][1];
+        if constexpr (DIM == 3) Nn += qm[ps][3] * n[ps][2];
+        wo[ps] = nodeLane ? Pv + Nn : T(0);
+        wi[ps] = Pv - Nn;
         for (int c = 0; c < NT; ++c) {
-            tt[c] = qm[1] * t[c][0] + qm[2] * t[c][1];
-            if constexpr (DIM == 3) tt[c] += qm[3] * t[c][2];
+            T v = qm[ps][1] * t[ps][c][0] + qm[ps][2] * t[ps][c][1];
+            if constexpr (DIM == 3) v += qm[ps][3] * t[ps][c][2];
+            tt[ps][c] = nodeLane ? v : T(0);
         }
-    }
-    const float wof = static_cast<float>(wo);
-    float ss = wof * wof;
-    for (int c = 0; c < NT; ++c) {
-        const float ttf = static_cast<float>(tt[c]);
-        ss += ttf * ttf;
-    }
+        // scale: RMS over the face's nodes of the outgoing and tangential characteristics
+        const float wof = static_cast<float>(wo[ps]);
+        float ss = wof * wof;
+        for (int c = 0; c < NT; ++c) {
+            const float ttf = static_cast<float>(tt[ps][c]);
+            ss += ttf * ttf;
+        }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
-    const float s = sqrtf(ss / NFP) + 1e-30f;
-    const float is = 1.0f / s;
-    if (lane < NFP) {
-        sx[lane] = wof * is;
-        for (int c = 0; c < NT; ++c) sx[(1 + c) * NFP + lane] = static_cast<float>(tt[c]) * is;
-    }
-    if (lane == 0) {
-        const float ic0 = static_cast<float>(T(1) / c0);
-        sx[DIM * NFP] = static_cast<float>(vn0) * ic0;
-        sx[DIM * NFP + 1] = static_cast<float>(vt0) * ic0;
+        for (int off = 8; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+        sc[ps] = sqrtf(ss / NFP) + 1e-30f;
+        const float is = 1.0f / sc[ps];
+        float* x = sx + (ps * 4 + slot) * XS;
+        if (nodeLane) {
+            x[node] = wof * is;
+            for (int c = 0; c < NT; ++c) x[(1 + c) * NFP + node] = static_cast<float>(tt[ps][c]) * is;
+        } else if (node == NFP) {
+            const float ic0 = static_cast<float>(T(1) / c0);
+            x[DIM * NFP] = static_cast<float>(vn0[ps]) * ic0;
+            x[DIM * NFP + 1] = static_cast<float>(vt0) * ic0;
+        } else if (node == NFP + 1) {
+            for (int i = NIN; i < XS; ++i) x[i] = 0.0f;
+        }
     }
     waveFence();
-    mlpForwardWave<NIN, NOUT>(net, sx, sa1, sa2, sy, lane);
-    if (lane < NFP) {
-        const T what = static_cast<T>(s * sy[lane]);
-        const T lam = vn0 - c0;
-        const T wip = (T(2) * lam * what - (lam + c0) * wi) / (lam - c0);
+    mlpForwardBatch<NIN, NOUT, FW, XS>(net, sx, bufA[wave], sx, bufY[wave], lane);
+#pragma unroll
+    for (int ps = 0; ps < NPASS; ++ps) {
+        const int64_t idx = f0 + ps * 4 + slot;
+        if (!nodeLane || idx >= a.nfaces) continue;
+        const T what = static_cast<T>(sc[ps] * bufY[wave][(ps * 4 + slot) * NOUT + node]);
+        const T lam = vn0[ps] - c0;
+        const T wip = (T(2) * lam * what - (lam + c0) * wi[ps]) / (lam - c0);
         T tp[NT];
-        for (int c = 0; c < NT; ++c) tp[c] = (vn0 >= T(0)) ? tt[c] : -(vn0 + c0) * tt[c] / (vn0 - c0);
-        const T Pp = T(0.5) * (wo + wip), Np_ = T(0.5) * (wo - wip);
+        for (int c = 0; c < NT; ++c)
+            tp[c] = (vn0[ps] >= T(0)) ? tt[ps][c] : -(vn0[ps] + c0) * tt[ps][c] / (vn0[ps] - c0);
+        const T Pp = T(0.5) * (wo[ps] + wip), Np_ = T(0.5) * (wo[ps] - wip);
         T qp[NF];
         qp[0] = a.ph.rho0 * c0 * Pp;
         for (int x = 0; x < DIM; ++x) {
-            T v = Np_ * n[x];
-            for (int c = 0; c < NT; ++c) v += tp[c] * t[c][x];
+            T v = Np_ * n[ps][x];
+            for (int c = 0; c < NT; ++c) v += tp[c] * t[ps][c][x];
             qp[1 + x] = v;
         }
-        if constexpr (DIM == 2) qp[3] = qm[3];
-        if constexpr (NF == 5) qp[4] = qm[4] + (qp[0] - qm[0]) * a.ph.ic2;
-        T* g = a.ghost + (idx * NFP + lane) * NF;
+        if constexpr (DIM == 2) qp[3] = qm[ps][3];
+        if constexpr (NF == 5) qp[4] = qm[ps][4] + (qp[0] - qm[ps][0]) * a.ph.ic2;
+        T* g = a.ghost + (idx * NFP + node) * NF;
 #pragma unroll
         for (int k = 0; k < NF; ++k) g[k] = qp[k];
     }

@@ -8,6 +8,12 @@ precision from the same initial state; mixed-precision error reported against FP
            absorbing elsewhere (group 1); CFL-limited dt.
   lee2d:   the 2D LEE case on [-1,1]^2 (96x96x2 triangles, p=3, v0=(-30,30), c0=100,
            reflecting, Gaussian pulse exp(-r^2/0.1)), CFL-limited dt.
+  closure3d: throughput of the learned absorbing closure at the size of the 3D closure
+           configuration's truncated domain: [-1,1]^3 as 64^3 x 6 jittered Kuhn tets (1,572,864
+           tets, 49,152 boundary faces), p=3, APE, mixed precision, v0=(30,0,0), a monopole and
+           8 observers; every boundary face absorbing (the reference's characteristic BC) or
+           learned: the zero network (the exact characteristic condition) and a seeded network
+           with a nonzero output layer (timing only, not trained).
 
     python dgfem-acoustic_amd/workloads.py --workload dense3d --steps 200
 """
@@ -59,6 +65,45 @@ def lee2d(n=96, seed=1):
 WORKLOADS = {"dense3d": dense3d, "lee2d": lee2d}
 
 
+def run_closure3d(steps, n=64, warmup=10, seed=5):
+    m = dg.box_mesh(3, n, 3, jitter=0.15, seed=seed)
+    X = m.node_coords.reshape(-1, 3)
+    out = dict(workload="closure3d", steps=steps,
+               describe=f"3D APE, {n}^3x6 jittered Kuhn tets (K={m.K}), p=3, mixed precision, boundary faces "
+                        "absorbing or learned")
+    nfp = 10
+    params = dg.to_bf16(dg.closure_init(nfp, seed=seed, out_scale=0.1, dim=3))
+    zero = dg.to_bf16(dg.closure_init(nfp, seed=seed, out_scale=0.0, dim=3))
+    for mode, code, net in (("characteristic", dg.DG_BC_ABSORBING, None), ("learned_zero", dg.DG_BC_LEARNED, zero),
+                            ("learned", dg.DG_BC_LEARNED, params)):
+        bc = dg.resolve_bc_sides(m, [(0x3F, code)])
+        s = dg.Solver(m.node_coords, m.element_vertices, 3, bc, v0=(30.0, 0.0, 0.0), rho0=RHO0, c0=340.0,
+                      precision=dg.DG_MIXED, steps_per_graph=20)
+        if net is not None:
+            s.set_closure(net)
+        s.add_monopole(0.0, 0.0, 0.0, 0.05, 0.1, 200.0, 0.0, 1e9)
+        for k in range(8):  # Fibonacci points on r = 0.8
+            z = 1.0 - (2 * k + 1) / 8.0
+            r, ph = np.sqrt(1.0 - z * z), np.pi * (3.0 - np.sqrt(5.0)) * k
+            s.add_observer(0.8 * r * np.cos(ph), 0.8 * r * np.sin(ph), 0.8 * z, 0.03)
+        info = s.info()
+        dt = info.min_inradius / ((340.0 + 30.0) * 7)
+        s.set_state(np.zeros((4, len(X))))
+        s.run_steps(warmup, dt)
+        s.set_state(np.zeros((4, len(X))))
+        r = s.run_steps(steps, dt)
+        stage, _, post = s.profile_steps(dt, 5)
+        dofs = info.num_elements * info.nodes_per_element * 4
+        out[mode] = dict(dof_updates_per_s=dofs * steps / (r.device_ms * 1e-3), ms_per_step=r.device_ms / steps,
+                         stage_ms=[float(v) for v in stage], post_ms=float(post), dt=dt, dofs=int(dofs),
+                         finite=bool(np.isfinite(r.observers).all()))
+        s.close()
+        print(f"# {mode}: {out[mode]['dof_updates_per_s']:.4e} DOF-updates/s, {out[mode]['ms_per_step']:.4f} ms/step",
+              flush=True)
+    out["closure_ms_per_step"] = out["learned"]["ms_per_step"] - out["characteristic"]["ms_per_step"]
+    return out
+
+
 def run(name, steps, warmup=10):
     w = WORKLOADS[name]()
     m = w["mesh"]
@@ -93,10 +138,11 @@ def run(name, steps, warmup=10):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="dense3d")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["closure3d"], default="dense3d")
     ap.add_argument("--steps", type=int, default=200)
     a = ap.parse_args(argv)
-    print(json.dumps(run(a.workload, a.steps)), flush=True)
+    res = run_closure3d(a.steps) if a.workload == "closure3d" else run(a.workload, a.steps)
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":
