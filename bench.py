@@ -112,11 +112,26 @@ def cpu_baseline(m, bc, dt, budget_s):
         if time.perf_counter() - t_start + times[-1] > budget_s:
             break
     per_stage = float(np.median(times))
+    # one single-threaded evaluation beside it
+    t0 = time.perf_counter()
+    o.rhs(u, dt, 1)
+    one = time.perf_counter() - t0
     dof = m.K * m.Np * 4
     return {"value": dof / (4.0 * per_stage), "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+            "value_1_core": dof / (4.0 * one), "cpu_model": cpu_model(),
             "sample": f"{len(times)} RHS evaluation(s) (one RK4 stage each) of the oracle on the full mesh "
                       f"({m.K} tets, p=3), median {per_stage:.2f} s per stage, x4 per step; oracle quadrature "
-                      f"{o.num_volume_points} points per element"}
+                      f"{o.num_volume_points} points per element; one more evaluation on 1 core: {one:.2f} s"}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
