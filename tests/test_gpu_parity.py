@@ -220,6 +220,38 @@ def test_mixed_precision_close_to_fp64(kernel, monkeypatch):
     assert er < 1e-4
 
 
+@pytest.mark.parametrize("dim,n,p,fields", [(3, 4, 3, dg.DG_FIELDS_LEE5), (3, 3, 5, dg.DG_FIELDS_LEE5),
+                                            (2, 12, 4, dg.DG_FIELDS_REF4), (2, 8, 5, dg.DG_FIELDS_REF4)])
+def test_mixed_precision_close_to_fp64_other_variants(dim, n, p, fields):
+    """Mixed precision against GPU FP64 beyond the 4-field tetrahedra: the 5-field LEE variant
+    (p = 3, 5) and triangles (p = 4, 5), absorbing walls, a monopole, 300 RK4 steps: pressure
+    rel L2 <= 1e-6."""
+    print(f"case mixed vs fp64 dim={dim} n={n} p={p} fields={fields}", flush=True)
+    groups = [(ALL3 if dim == 3 else ALL2, 0)]
+    v0 = (30.0, -10.0, 5.0) if dim == 3 else (30.0, -10.0, 0.0)
+    m, bc, gn = make_case(dim, n, p, groups, jitter=0.15, seed=9)
+    nf = 5 if fields == dg.DG_FIELDS_LEE5 else 4
+    u0 = gaussian_state(m, size=0.02, nf=nf, c0=340.0)
+    probe = dg.Solver(m.node_coords, m.element_vertices, p, bc, v0=v0, rho0=RHO0, c0=340.0, fields=fields)
+    dt = min(0.5 * probe.info().min_inradius / ((340.0 + 35.0) * (2 * p + 1)), 1e-5)
+    probe.close()
+    res = []
+    for prec in (dg.DG_FP64, dg.DG_MIXED):
+        s = dg.Solver(m.node_coords, m.element_vertices, p, bc, v0=v0, rho0=RHO0, c0=340.0, fields=fields,
+                      precision=prec)
+        s.set_state(u0)
+        s.add_monopole(0.1, 0.0, 0.0, 0.2, 0.1, 300.0, 0.0, 1.0)
+        s.add_observer(-0.4, 0.2, 0.0, 0.3)
+        r = s.run_steps(300, dt)
+        res.append((s.get_state(), r.observers))
+        s.close()
+    e = rel_l2(res[1][0][0], res[0][0][0])
+    eo = rel_l2(res[1][1][:, :, 0], res[0][1][:, :, 0])
+    print("mixed rel L2 p:", e, " observer:", eo)
+    assert np.abs(res[0][0][0]).max() > 1e-3
+    assert e < 1e-6 and eo < 1e-6
+
+
 def test_lee5_keeps_isentropic_density():
     print("case lee5 invariant", flush=True)
     groups = [(0x10, 1), (0x2F, 0)]
