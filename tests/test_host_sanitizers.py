@@ -1,0 +1,92 @@
+"""The host side of the library (configuration and Gmsh readers, BC rule, source tables, formula
+evaluator, box meshes, DG precompute for every (dim, order)) built with g++ under
+AddressSanitizer and UndefinedBehaviorSanitizer and run over the reference's configuration
+files and meshes, the formula corpus and the source tables (tests/sanitize/host_driver.cpp).
+Any sanitizer report aborts the driver."""
+import concurrent.futures as cf
+import glob
+import json
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CASES = os.path.join(HERE, "golden", "ref_cases")
+FLAGS = ["-std=c++17", "-g", "-O1", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+         "-fno-omit-frame-pointer", "-I" + os.path.join(ROOT, "include")]
+
+CONF = """[Config]
+meshFileName = {mesh}
+timeStart=0
+timeEnd=0.01
+timeStep=1e-4
+timeRate=0.005
+elementType=Lagrange
+timeIntMethod=Runge-Kutta
+{bc}
+numThreads=1
+v0_x = 10
+v0_y = -5
+v0_z = 0
+rho0 = 1.2
+c0 = 343
+source2 = dipole, 1,2,0, 0.4, 2,10,0.5,5
+source10 = quadrupole, -1,0,0, 0.6, 3,20,0,1
+source3 = formula, "0.1 * sin(2 * pi * 50 * t) + exp(-t)", 0.5,0.5,0.0, 0.1, 0.2
+source4 = file,"data/data.csv", 0.0,0.0,0.0, 0.1
+source5 = file,"data/data1.wav", 0.2,0.0,0.0, 0.1
+observer1 = 1,1,0,0.5
+observer2 = 0,0,0,0.2
+initialCondtition1 = gaussian, 0,0,0,1,2
+"""
+
+MESHES = [("doc/2d/square_reflection.msh", "Absorbing = Absorbing\nReflecting = Reflecting"),
+          ("doc/2d/square2.msh", "Abs = Reflecting"),
+          ("doc/2d/room.msh", "Absorbing = Absorbing"),
+          ("doc/3d/sphere.msh", "Absorbing = Absorbing")]
+
+
+@pytest.fixture(scope="module")
+def driver(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ is not available")
+    d = tmp_path_factory.mktemp("sanitize")
+    srcs = [os.path.join(HERE, "sanitize", "host_driver.cpp")] + sorted(
+        glob.glob(os.path.join(ROOT, "dgfem-acoustic_amd", "csrc", "host", "*.cpp")))
+
+    def compile_one(src):
+        obj = str(d / (os.path.basename(src) + ".o"))
+        r = subprocess.run(["g++", *FLAGS, "-c", src, "-o", obj], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        return obj
+
+    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        objs = list(ex.map(compile_one, srcs))
+    exe = str(d / "host_driver")
+    r = subprocess.run(["g++", *FLAGS, *objs, "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def test_host_code_is_clean_under_sanitizers(driver, tmp_path):
+    for rel in ["cases/2d_hilltop/2d_hilltop.json", "cases/2d_hilltop/2d_hilltop.conf",
+                "cases/2d_hilltop/2d_hilltop_fs.msh", "data/data.csv", "data/data1.wav"] + [m for m, _ in MESHES]:
+        os.makedirs(tmp_path / os.path.dirname(rel), exist_ok=True)
+        shutil.copyfile(os.path.join(CASES, rel), tmp_path / rel)
+    configs = ["cases/2d_hilltop/2d_hilltop.json", "cases/2d_hilltop/2d_hilltop.conf"]
+    for i, (mesh, bc) in enumerate(MESHES):
+        (tmp_path / f"case{i}.conf").write_text(CONF.format(mesh=mesh, bc=bc))
+        configs.append(f"case{i}.conf")
+    corpus = json.load(open(os.path.join(HERE, "golden", "eqedit_corpus.json")))
+    (tmp_path / "formulas.txt").write_text("".join(f"{c['t']!r}\t{c['expr']}\n" for c in corpus))
+    args = [driver, "--formulas", "formulas.txt", "--table", "data/data.csv", "--table", "data/data1.wav"]
+    for c in configs:
+        args += ["--config", c]
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1")
+    r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, env=env, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "runtime error" not in r.stderr and "ERROR: AddressSanitizer" not in r.stderr, r.stderr[-4000:]
+    assert f"{len(corpus)} formulas, 2 tables, {len(configs)} configs, 0 failures" in r.stdout, r.stdout
