@@ -119,7 +119,11 @@ typedef int (*dg_step_fn)(void* user, const dg_step_record* rec);
 /* Save steps (src/solver.cpp:421-453): state [fields][K*Np] before the source overwrite. */
 typedef int (*dg_save_fn)(void* user, int64_t step, double t, const double* state);
 
-/* solver::rungeKutta / solver::forwardEuler time loop. Callbacks may be NULL. */
+/* solver::rungeKutta / solver::forwardEuler time loop. Callbacks may be NULL. Steps run on the
+   device in batches (at most steps_per_graph steps, never across a save step); the step records
+   of a batch are delivered after it. When a step callback returns nonzero (DG_ERR_CALLBACK) or a
+   residual is non-finite (DG_ERR_NONFINITE), *steps_done counts the steps delivered up to and
+   including that step, and the state held by the context is the one at the end of its batch. */
 DG_API dg_status dg_run(dg_context* ctx, const dg_run_params* prm, dg_step_fn on_step, dg_save_fn on_save,
                         void* user, int64_t* steps_done);
 

@@ -1,0 +1,109 @@
+"""Behaviour of the C ABI across calls, on the GPU: successive runs on one context (changed dt,
+changed integrator) continue from the state the previous run left, as the oracle does for the
+same sequence; state round trips; status codes of invalid calls and of a failing callback."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import dgfem_amd as dg
+from cases import ALL2, gaussian_state, make_case, oracle_for, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+RHO0, C0 = 1.225, 100.0
+V0 = (-30.0, 20.0, 0.0)
+
+
+def _case():
+    groups = [(0x3, 1), (0xC, 0)]
+    m, bc, gn = make_case(2, 6, 3, groups, jitter=0.15, seed=12)
+    return m, bc, oracle_for(m, gn, groups, V0, RHO0, C0)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_successive_runs_continue_the_state(use_graph):
+    print(f"case successive runs graph={use_graph}", flush=True)
+    m, bc, o = _case()
+    u = gaussian_state(m, 0.1, -0.1, 0.0, 0.05)
+    s = dg.Solver(m.node_coords, m.element_vertices, 3, bc, v0=V0, rho0=RHO0, c0=C0, use_graph=use_graph,
+                  steps_per_graph=6)
+    s.set_state(u)
+    t = 0.0
+    for dt, steps, integ in ((2e-5, 30, dg.DG_RK4), (1.3e-5, 17, dg.DG_RK4), (4e-6, 9, dg.DG_EULER),
+                             (2e-5, 11, dg.DG_RK4)):
+        t_end = t + (steps - 0.5) * dt
+        ref = o.run(u, t, t_end, dt, 1e9, integrator=0 if integ == dg.DG_RK4 else 1)
+        r = s.run(t, t_end, dt, 1e9, integrator=integ)
+        assert r.steps == ref["steps"] == steps
+        np.testing.assert_allclose(r.residuals, ref["residuals"], rtol=1e-8, atol=1e-300)
+        got = s.get_state()
+        for f in range(3):
+            assert rel_l2(got[f], ref["u"][f]) < 1e-10, (dt, integ, f, rel_l2(got[f], ref["u"][f]))
+        u = ref["u"]
+        t = t + steps * dt
+    s.close()
+
+
+def test_state_round_trips():
+    print("case state round trip", flush=True)
+    m, bc, _ = _case()
+    rng = np.random.default_rng(4)
+    u = rng.normal(size=(4, m.K * m.Np)) * np.exp(rng.uniform(-20, 20, size=(4, m.K * m.Np)))
+    s64 = dg.Solver(m.node_coords, m.element_vertices, 3, bc, v0=V0, rho0=RHO0, c0=C0)
+    s64.set_state(u)
+    assert np.array_equal(s64.get_state(), u)
+    smx = dg.Solver(m.node_coords, m.element_vertices, 3, bc, v0=V0, rho0=RHO0, c0=C0, precision=dg.DG_MIXED)
+    smx.set_state(u)
+    # compensated pair: the FP32 leading word plus the FP32 rounding error of it
+    assert np.all(np.abs(smx.get_state() - u) <= 2.0 ** -46 * np.abs(u))
+    s64.close()
+    smx.close()
+
+
+def test_zero_steps_and_invalid_calls():
+    print("case invalid calls", flush=True)
+    m, bc, _ = _case()
+    u = gaussian_state(m)
+    s = dg.Solver(m.node_coords, m.element_vertices, 3, bc, v0=V0, rho0=RHO0, c0=C0)
+    s.set_state(u)
+    r = s.run(1.0, 0.5, 1e-5, 1e9)
+    assert r.steps == 0 and np.array_equal(s.get_state(), u)
+    for dt in (0.0, -1e-5, float("nan")):
+        with pytest.raises(dg.DGError) as e:
+            s.run(0.0, 1e-3, dt, 1e9)
+        assert e.value.code == dg.DG_ERR_INVALID
+    with pytest.raises(dg.DGError) as e:
+        s.run(0.0, 1e-3, 1e-5, 1e9, integrator=7)
+    assert e.value.code == dg.DG_ERR_INVALID
+    with pytest.raises(dg.DGError) as e:
+        s.set_closure(np.zeros(dg.closure_num_params(4), dtype=np.uint16))
+    assert e.value.code == dg.DG_ERR_INVALID
+    with pytest.raises(dg.DGError) as e:
+        s.face_traces(np.array([m.K], dtype=np.int64), np.array([0], dtype=np.int32))
+    assert e.value.code == dg.DG_ERR_INVALID
+    assert np.array_equal(s.get_state(), u)
+    s.close()
+
+
+def test_failing_step_callback_stops_the_run():
+    print("case failing callback", flush=True)
+    m, bc, _ = _case()
+    s = dg.Solver(m.node_coords, m.element_vertices, 3, bc, v0=V0, rho0=RHO0, c0=C0, steps_per_graph=5)
+    s.set_state(gaussian_state(m))
+    seen = []
+
+    def step_cb(_user, recp):
+        seen.append(recp.contents.step)
+        return 1 if recp.contents.step == 7 else 0
+
+    def save_cb(_user, step, t, state):
+        return 0
+
+    prm = dg.RunParams(0.0, 1.0, 1e-5, 1e9, dg.DG_RK4, 1, -1)
+    done = C.c_int64()
+    st = s._lib.dg_run(s._h, C.byref(prm), dg.STEP_FN(step_cb), dg.SAVE_FN(save_cb), None, C.byref(done))
+    assert st == dg.DG_ERR_CALLBACK
+    assert done.value == 8 and seen == list(range(8))
+    assert "callback" in s._lib.dg_last_error(s._h).decode()
+    s.close()
