@@ -107,3 +107,52 @@ def test_failing_step_callback_stops_the_run():
     assert done.value == 8 and seen == list(range(8))
     assert "callback" in s._lib.dg_last_error(s._h).decode()
     s.close()
+
+
+def test_independent_contexts_interleaved_and_concurrent():
+    """Contexts share nothing: two solvers on different meshes and precisions, run interleaved
+    and then from two host threads at once, end in the same states as when run alone."""
+    import threading
+    print("case two contexts", flush=True)
+    m1, bc1, _ = _case()
+    m2, bc2, _ = make_case(3, 3, 2, [(0x3F, 0)], jitter=0.15, seed=13)
+    u1, u2 = gaussian_state(m1, size=0.05), gaussian_state(m2, size=0.05)
+
+    def solver(m, bc, p, prec):
+        s = dg.Solver(m.node_coords, m.element_vertices, p, bc, v0=V0, rho0=RHO0, c0=C0, precision=prec,
+                      steps_per_graph=4)
+        return s
+
+    alone = []
+    for m, bc, p, prec, u in ((m1, bc1, 3, dg.DG_FP64, u1), (m2, bc2, 2, dg.DG_MIXED, u2)):
+        s = solver(m, bc, p, prec)
+        s.set_state(u)
+        s.run_steps(40, 1e-5)
+        alone.append(s.get_state())
+        s.close()
+    a, b = solver(m1, bc1, 3, dg.DG_FP64), solver(m2, bc2, 2, dg.DG_MIXED)
+    a.set_state(u1)
+    b.set_state(u2)
+    for _ in range(4):
+        a.run_steps(10, 1e-5)
+        b.run_steps(10, 1e-5)
+    assert np.array_equal(a.get_state(), alone[0]) and np.array_equal(b.get_state(), alone[1])
+    a.set_state(u1)
+    b.set_state(u2)
+    errors = []
+
+    def work(s):
+        try:
+            s.run_steps(40, 1e-5)
+        except Exception as ex:  # reported below
+            errors.append(ex)
+
+    th = [threading.Thread(target=work, args=(s,)) for s in (a, b)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors
+    assert np.array_equal(a.get_state(), alone[0]) and np.array_equal(b.get_state(), alone[1])
+    a.close()
+    b.close()
