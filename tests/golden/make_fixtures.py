@@ -39,6 +39,10 @@ DATA_FILES = [
     "doc/2d/square2.msh",
     "doc/2d/room.msh",
     "doc/3d/sphere.msh",
+    "doc/2d/square.msh",
+    "doc/2d/disk.msh",
+    "doc/config/Square.conf",
+    "doc/config/Square.json",
     "data/data.csv",
     "data/data1.wav",
     "residuals.csv",

@@ -219,22 +219,45 @@ def _mesh_summary(tmp_path, monkeypatch, rel, bcs):
 @pytest.mark.parametrize("rel,tag_types,order,npe", [
     ("doc/2d/square2.msh", {9: 0}, 2, 6),     # second-order triangles, physical 'Abs'
     ("doc/3d/sphere.msh", {1: 0}, 1, 4),      # linear tetrahedra, physical surface 'Absorbing'
+    ("doc/2d/square.msh", {9: 0}, 2, 6),      # second-order triangles, physical 'Absorbing'
+    ("doc/2d/disk.msh", {2: 1}, 1, 3),        # linear triangles, physical 'Reflecting'
 ])
 def test_reference_meshes_against_python_reader(tmp_path, monkeypatch, rel, tag_types, order, npe):
     """Element count, order, node count per element and boundary faces of the reference's own
-    p=2 triangle mesh and tetrahedral mesh, against the independent Python reader and the
-    first-node rule (src/Mesh.cpp:579-605)."""
+    triangle meshes (order 1 and 2) and tetrahedral mesh, against the independent Python reader
+    and the first-node rule (src/Mesh.cpp:579-605)."""
     from msh_py import first_node_bc_any, read_msh41
     m = read_msh41(os.path.join(CASES, rel))
-    c = _mesh_summary(tmp_path, monkeypatch, rel, ["Abs = Absorbing", "Absorbing = Absorbing"])
+    c = _mesh_summary(tmp_path, monkeypatch, rel, ["Abs = Absorbing", "Absorbing = Absorbing",
+                                                   "Reflecting = Reflecting"])
     assert c["dim"] == m["dim"] and c["order"] == order and c["nodes_per_element"] == npe
     assert c["elements"] == len(m["tags"]) and m["tags"].shape[1] == npe
     bc = first_node_bc_any(m["verts"], m["dim"], m["groups"], tag_types)
     assert c["bc_counts"]["absorbing"] == sum(1 for v in bc.values() if v == 0)
-    assert c["bc_counts"]["reflecting"] == sum(1 for v in bc.values() if v == 1) == 0
-    assert [n for _, n in c["physical_bcs"]] == ["Absorbing"]
+    assert c["bc_counts"]["reflecting"] == sum(1 for v in bc.values() if v == 1)
+    assert [n for _, n in c["physical_bcs"]] == [("Absorbing", "Reflecting")[tag_types[t]] for t in sorted(tag_types)]
     # a closed boundary: every boundary face is typed and the count is exactly the unpaired faces
-    assert len(bc) == sum(c["bc_counts"].values())
+    assert len(bc) == sum(c["bc_counts"].values()) > 0
+
+
+def test_reference_example_configs(tmp_path, monkeypatch):
+    """The reference's own example configuration in both grammars (doc/config/Square.conf and
+    Square.json, on doc/2d/square2.msh)."""
+    _stage_case(tmp_path, "doc/config/Square.conf", "doc/config/Square.json", "doc/2d/square2.msh")
+    monkeypatch.chdir(tmp_path)
+    c = dg.case_summary("doc/config/Square.conf")
+    j = dg.case_summary("doc/config/Square.json")
+    for x, t_end, threads, duration in ((c, 0.1, 12, 0.1), (j, 0.05, 16, 0.05)):
+        assert x["dim"] == 2 and x["order"] == 2 and x["elements"] == 3706
+        assert x["time"] == [0.0, t_end, 5e-05, 0.001]
+        assert x["integrator"] == "Runge-Kutta" and x["num_threads"] == threads
+        assert x["mean_flow"] == [30.0, 0.0, 0.0, 1.225, 100.0]
+        assert x["bc_counts"] == {"absorbing": 160, "reflecting": 0}
+        assert x["observers"] == [[2.11792, 0.00340081, 0.0, 0.1]]
+        assert x["initial_conditions"] == []
+        (s,) = x["sources"]
+        assert s["kind"] == 1 and s["formula"] == "0.1*sin(2*pi*50*t)" and s["duration"] == duration
+        assert (s["x"], s["y"], s["z"], s["size"]) == (0.0, 0.0, 0.0, 0.1)
 
 
 def test_reference_msh40_mesh(tmp_path, monkeypatch):

@@ -46,7 +46,8 @@ initialCondtition1 = gaussian, 0,0,0,1,2
 MESHES = [("doc/2d/square_reflection.msh", "Absorbing = Absorbing\nReflecting = Reflecting"),
           ("doc/2d/square2.msh", "Abs = Reflecting"),
           ("doc/2d/room.msh", "Absorbing = Absorbing"),
-          ("doc/3d/sphere.msh", "Absorbing = Absorbing")]
+          ("doc/3d/sphere.msh", "Absorbing = Absorbing"),
+          ("doc/2d/disk.msh", "Reflecting = Reflecting")]
 
 
 @pytest.fixture(scope="module")
@@ -73,10 +74,12 @@ def driver(tmp_path_factory):
 
 def test_host_code_is_clean_under_sanitizers(driver, tmp_path):
     for rel in ["cases/2d_hilltop/2d_hilltop.json", "cases/2d_hilltop/2d_hilltop.conf",
-                "cases/2d_hilltop/2d_hilltop_fs.msh", "data/data.csv", "data/data1.wav"] + [m for m, _ in MESHES]:
+                "cases/2d_hilltop/2d_hilltop_fs.msh", "data/data.csv", "data/data1.wav", "doc/config/Square.conf",
+                "doc/config/Square.json"] + [m for m, _ in MESHES]:
         os.makedirs(tmp_path / os.path.dirname(rel), exist_ok=True)
         shutil.copyfile(os.path.join(CASES, rel), tmp_path / rel)
-    configs = ["cases/2d_hilltop/2d_hilltop.json", "cases/2d_hilltop/2d_hilltop.conf"]
+    configs = ["cases/2d_hilltop/2d_hilltop.json", "cases/2d_hilltop/2d_hilltop.conf", "doc/config/Square.conf",
+               "doc/config/Square.json"]
     for i, (mesh, bc) in enumerate(MESHES):
         (tmp_path / f"case{i}.conf").write_text(CONF.format(mesh=mesh, bc=bc))
         configs.append(f"case{i}.conf")
