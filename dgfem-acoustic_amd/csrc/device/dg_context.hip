@@ -54,7 +54,8 @@ struct dg_context {
     int32_t *nbr = nullptr, *iops = nullptr;
     uint8_t* code = nullptr;
     double* resPartial = nullptr;
-    int64_t nblocks = 0;
+    double* resGroup = nullptr;  // per-group residual sums when nblocks > 4 kResGroup
+    int64_t nblocks = 0, nResGroups = 0;
     // sources / observers (host definitions + device tables)
     std::vector<std::vector<int64_t>> srcNodes;
     std::vector<std::array<double, 5>> srcParams;  // kind, amp, freq, phase, duration
@@ -297,6 +298,12 @@ void launchPost(dg_context* c) {
     dgk::PostArgs p{};
     p.resPartial = c->resPartial;
     p.nblocks = c->nblocks;
+    if (c->resGroup && c->curResid) {
+        hipLaunchKernelGGL(dgk::residualGroupKernel, dim3(static_cast<unsigned>(c->nResGroups)), dim3(256), 0, c->stream,
+                           static_cast<const double*>(c->resPartial), c->nblocks, c->resGroup);
+        p.resPartial = c->resGroup;
+        p.nblocks = c->nResGroups;
+    }
     p.invN = 1.0 / static_cast<double>(c->K * c->Np);
     p.obsIdx = c->dObsIdx;
     p.obsW = c->dObsW;
@@ -668,6 +675,7 @@ void destroyContext(dg_context* c) {
     devFree(c->iops);
     devFree(c->code);
     devFree(c->resPartial);
+    devFree(c->resGroup);
     devFree(c->dT);
     devFree(c->dV);
     devFree(c->dVF);
@@ -815,6 +823,10 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         }
         c->resPartial = devAlloc<double>(c, static_cast<size_t>(c->nblocks) * 5);
         HIPCHK(hipMemset(c->resPartial, 0, static_cast<size_t>(c->nblocks) * 5 * sizeof(double)));
+        if (c->nblocks > 4 * dgk::kResGroup) {  // below that the extra launch costs what it saves
+            c->nResGroups = (c->nblocks + dgk::kResGroup - 1) / dgk::kResGroup;
+            c->resGroup = devAlloc<double>(c, static_cast<size_t>(c->nResGroups) * 5);
+        }
         c->ringSlots = 4096;
         c->resRing = devAlloc<double>(c, c->ringSlots * 5);
         c->nonfinite = devAlloc<int>(c, 1);

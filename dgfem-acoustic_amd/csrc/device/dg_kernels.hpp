@@ -1412,6 +1412,35 @@ __global__ __launch_bounds__(256) void sourceKernel(SourceArgs s, const int64_t*
     applySources<T, COMP>(s, *counter, u, ulo);
 }
 
+// Residual partial sums of many stage blocks folded into groups of kResGroup blocks (one block
+// of 256 threads per group; thread t takes blocks t, t + 256, ... of its group in order, then
+// rows of 16 lanes and the rows in order), so that the single-block end-of-step kernel reads one
+// partial per group. Used when the stage grid has more than 4 kResGroup blocks.
+constexpr int kResGroup = 4096;
+
+[[maybe_unused]] static __global__ __launch_bounds__(256) void residualGroupKernel(const double* __restrict__ part, int64_t n,
+                                                           double* __restrict__ out) {
+    __shared__ double sRed[256 / 16][5];
+    const int64_t g0 = static_cast<int64_t>(blockIdx.x) * kResGroup;
+    const int64_t g1 = (g0 + kResGroup < n) ? g0 + kResGroup : n;
+    constexpr int U = kResGroup / 256;
+    double v[U][5];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+        const int64_t b = g0 + threadIdx.x + static_cast<int64_t>(q) * 256;
+        const int64_t bc = b < g1 ? b : g0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) v[q][k] = part[bc * 5 + k];
+    }
+    double acc[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < U; ++q)
+        if (g0 + threadIdx.x + static_cast<int64_t>(q) * 256 < g1)
+#pragma unroll
+            for (int k = 0; k < 5; ++k) acc[k] += v[q][k];
+    blockResiduals<256>(acc, sRed, out + static_cast<int64_t>(blockIdx.x) * 5);
+}
+
 struct PostArgs {
     const double* resPartial;   // [nblocks][5] from the final stage
     int64_t nblocks;

@@ -156,3 +156,26 @@ def test_independent_contexts_interleaved_and_concurrent():
     assert np.array_equal(a.get_state(), alone[0]) and np.array_equal(b.get_state(), alone[1])
     a.close()
     b.close()
+
+
+def test_residuals_of_a_large_grid_match_the_host_sum():
+    """More than 16,384 stage blocks (56^3 x 6 linear tets, 64 per block): the residual partial
+    sums are folded in groups before the end-of-step kernel. The last step's residuals against
+    the snapshot taken at step 0 equal the mean squared differences summed on the host from the
+    returned states."""
+    print("case residuals large grid 56^3x6 p1", flush=True)
+    m = dg.box_mesh(3, 56, 1, jitter=0.1, seed=14)
+    bc = dg.resolve_bc_sides(m, [(0x3F, dg.DG_BC_ABSORBING)])
+    c0 = 340.0
+    s = dg.Solver(m.node_coords, m.element_vertices, 1, bc, v0=(30.0, 0.0, 0.0), rho0=RHO0, c0=c0)
+    assert m.K // 64 > 16384
+    u0 = gaussian_state(m, 0.1, 0.0, -0.1, 0.05)
+    s.set_state(u0)
+    r = s.run_steps(3, 2e-6)
+    u = s.get_state()
+    d = u - u0
+    host = [np.mean(d[0] ** 2), np.mean((d[0] / c0 ** 2) ** 2), np.mean(d[1] ** 2), np.mean(d[2] ** 2),
+            np.mean(d[3] ** 2)]
+    np.testing.assert_allclose(r.residuals[-1], host, rtol=1e-10, atol=0)
+    assert min(host) > 0
+    s.close()
