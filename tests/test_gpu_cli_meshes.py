@@ -216,3 +216,27 @@ def test_cli_snapshot_matches_oracle(case):
     assert np.abs(cell_array("Pressure [Pa]") - p_ref).max() < 1e-10 * sc
     vs = np.abs(v_ref).max()
     assert np.abs(cell_array("Velocity [m/s]").reshape(K, 3) - v_ref).max() < 1e-10 * vs
+
+
+def test_cli_mixed_precision_close_to_fp64(case):
+    """DGFEM_PRECISION=mixed selects the mixed-precision path in the command-line driver: its
+    observer series stay within 1e-5 relative L2 of the FP64 run's (both printed with 6
+    significant digits)."""
+    d = case["dir"]
+    mixed = d / "mixed"
+    if not mixed.exists():
+        shutil.copytree(d, mixed, ignore=shutil.ignore_patterns("results", "residuals.csv"))
+    print(f"case cli {case['name']} conf mixed", flush=True)
+    env = dict(os.environ, DGFEM_PRECISION="mixed")
+    r = subprocess.run([EXE, "case.conf"], cwd=mixed, capture_output=True, text=True, timeout=300, env=env)
+    assert r.returncode == 0, r.stderr
+    for j in range(len(case["observers"])):
+        def series(path):
+            rows = open(path).read().splitlines()[1:]
+            return np.array([[float(x) for x in row.split(";")] for row in rows])
+        a = series(d / "results" / f"observers{j + 1}.txt")
+        b = series(mixed / "results" / f"observers{j + 1}.txt")
+        assert a.shape == b.shape
+        assert np.array_equal(a[:, 0], b[:, 0])
+        col = 2  # pressure (rows are time; density; pressure; ... for both integrators)
+        assert np.linalg.norm(b[:, col] - a[:, col]) <= 1e-5 * np.linalg.norm(a[:, col])
