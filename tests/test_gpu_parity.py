@@ -176,6 +176,40 @@ def test_euler_matches_oracle():
     np.testing.assert_allclose(r.residuals, ref["residuals"], rtol=1e-8, atol=1e-300)
 
 
+@pytest.mark.parametrize("kernel", ["valu", "mm"])
+def test_euler_3d_with_sources_matches_oracle(kernel, monkeypatch):
+    """Forward Euler on tetrahedra (p = 3, reflecting floor, absorbing elsewhere) with a monopole
+    and an observer, 30 steps in batches of 7, against the oracle; the mixed-precision Euler path
+    within 1e-6 of FP64."""
+    print(f"case euler 3d p3 kernel={kernel}", flush=True)
+    monkeypatch.setenv("DGFEM_KERNEL", kernel)
+    groups = [(0x10, 1), (0x2F, 0)]
+    v0 = (25.0, -10.0, 5.0)
+    m, bc, gn = make_case(3, 4, 3, groups, jitter=0.15, seed=15)
+    o = oracle_for(m, gn, groups, v0, RHO0, 340.0)
+    u0 = gaussian_state(m, 0.1, -0.2, 0.1, 0.03)
+    src = dict(kind=0, x=-0.2, y=0.1, z=0.0, size=0.3, amp=0.05, freq=400.0, phase=0.3, duration=1.0)
+    obs = [(0.3, 0.3, -0.2, 0.3)]
+    dt, steps = 1e-6, 30
+    ref = o.run(u0, 0.0, (steps - 0.5) * dt, dt, 1e9, integrator=1, sources=[src], observers=obs)
+    out = {}
+    for prec in (dg.DG_FP64, dg.DG_MIXED):
+        s = dg.Solver(m.node_coords, m.element_vertices, 3, bc, v0=v0, rho0=RHO0, c0=340.0, precision=prec,
+                      steps_per_graph=7)
+        s.set_state(u0)
+        assert s.add_monopole(-0.2, 0.1, 0.0, 0.3, 0.05, 400.0, 0.3, 1.0) > 0
+        s.add_observer(*obs[0])
+        r = s.run(0.0, (steps - 0.5) * dt, dt, 1e9, integrator=dg.DG_EULER)
+        out[prec] = (s.get_state(), r)
+        s.close()
+    u, r = out[dg.DG_FP64]
+    for f in range(4):
+        assert rel_l2(u[f], ref["u"][f]) < 1e-10, (f, rel_l2(u[f], ref["u"][f]))
+    np.testing.assert_allclose(r.residuals, ref["residuals"], rtol=1e-8, atol=1e-300)
+    np.testing.assert_allclose(r.observers, ref["observers"], rtol=1e-8, atol=1e-14)
+    assert rel_l2(out[dg.DG_MIXED][0][0], u[0]) < 1e-6
+
+
 def test_graph_replay_equals_eager():
     print("case graph vs eager", flush=True)
     groups = [(ALL3, 0)]
