@@ -255,23 +255,24 @@ __device__ inline void mlpForwardBatch(const float* net, const float* sx, float*
     waveFence();
 }
 
-// kClosureFacesPerWave faces per wave, kClosureWaves waves per block: the network is staged
-// in LDS once per block and every weight read serves a wave's faces together.
-constexpr int kClosureFacesPerWave = 8;
+// FW faces per wave (4 or 8), kClosureWaves waves per block: the network is staged in LDS
+// once per block and every weight read serves a wave's faces together. With few faces (a 2D
+// truncation boundary) 4 faces per wave spread the work over twice the waves.
 constexpr int kClosureWaves = 4;
-constexpr int kClosureFacesPerBlock = kClosureFacesPerWave * kClosureWaves;
 constexpr int kClosureBlock = 64 * kClosureWaves;
+constexpr int kClosureSmall = 4096;  // at most this many faces: 4 faces per wave, else 8
 
 // Exterior states of the learned faces for one stage. Face-node work runs with lane =
 // (face slot, node): 16 lanes per face (NFP <= 16), 4 faces per pass.
-template <int DIM, int P, int NF, typename T>
+template <int DIM, int P, int NF, typename T, int FW>
 __global__ __launch_bounds__(kClosureBlock) void closureKernel(ClosureArgs<T> a) {
     using S = dgk::Sizes<DIM, P>;
     using L = dgk::OpLayout<DIM, P>;
     constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC;
     constexpr int NT = DIM - 1;  // tangential directions
     constexpr int NIN = DIM * NFP + 2, NOUT = NFP;
-    constexpr int FW = kClosureFacesPerWave, NPASS = FW / 4;
+    constexpr int NPASS = FW / 4;
+    static_assert(FW % 4 == 0, "4 faces per pass");
     constexpr int XS = (NIN + 3) & ~3;
     static_assert(NFP <= 16 && XS <= kH, "closure lane layout");
     using N = NetLds<NIN, NOUT>;
@@ -282,7 +283,7 @@ __global__ __launch_bounds__(kClosureBlock) void closureKernel(ClosureArgs<T> a)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int slot = lane >> 4, node = lane & 15;
     const bool nodeLane = node < NFP;
-    const int64_t f0 = static_cast<int64_t>(blockIdx.x) * kClosureFacesPerBlock + wave * FW;
+    const int64_t f0 = static_cast<int64_t>(blockIdx.x) * (kClosureWaves * FW) + wave * FW;
     // face records of this lane's faces, issued with the network loads
     int64_t ef[NPASS];
 #pragma unroll

@@ -333,9 +333,17 @@ void launchClosure(dg_context* c, const void* qin) {
     a.w = c->dClosureW;
     a.nfaces = c->nLearned;
     a.ph = makePhysics<T>(c->phys);
-    const int64_t nb = (c->nLearned + dgc::kClosureFacesPerBlock - 1) / dgc::kClosureFacesPerBlock;
-    hipLaunchKernelGGL((dgc::closureKernel<DIM, P, NF, T>), dim3(static_cast<unsigned>(nb)), dim3(dgc::kClosureBlock), 0,
-                       c->stream, a);
+    if (c->nLearned <= dgc::kClosureSmall) {
+        constexpr int kPerBlock = dgc::kClosureWaves * 4;
+        const int64_t nb = (c->nLearned + kPerBlock - 1) / kPerBlock;
+        hipLaunchKernelGGL((dgc::closureKernel<DIM, P, NF, T, 4>), dim3(static_cast<unsigned>(nb)),
+                           dim3(dgc::kClosureBlock), 0, c->stream, a);
+    } else {
+        constexpr int kPerBlock = dgc::kClosureWaves * 8;
+        const int64_t nb = (c->nLearned + kPerBlock - 1) / kPerBlock;
+        hipLaunchKernelGGL((dgc::closureKernel<DIM, P, NF, T, 8>), dim3(static_cast<unsigned>(nb)),
+                           dim3(dgc::kClosureBlock), 0, c->stream, a);
+    }
 }
 
 template <int DIM, int P, int NF, typename T>

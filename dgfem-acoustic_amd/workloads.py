@@ -14,6 +14,8 @@ precision from the same initial state; mixed-precision error reported against FP
            8 observers; every boundary face absorbing (the reference's characteristic BC) or
            learned: the zero network (the exact characteristic condition) and a seeded network
            with a nonzero output layer (timing only, not trained).
+  closure2d: the same at the 2D configuration's truncated size: [-1,1]^2 as 100^2 x 2 jittered
+           triangles (20,000 triangles, 400 boundary faces), p=4.
 
     python dgfem-acoustic_amd/workloads.py --workload dense3d --steps 200
 """
@@ -65,29 +67,34 @@ def lee2d(n=96, seed=1):
 WORKLOADS = {"dense3d": dense3d, "lee2d": lee2d}
 
 
-def run_closure3d(steps, n=64, warmup=10, seed=5):
-    m = dg.box_mesh(3, n, 3, jitter=0.15, seed=seed)
+def run_closure(dim, steps, warmup=10, seed=5):
+    n, p = (64, 3) if dim == 3 else (100, 4)
+    m = dg.box_mesh(dim, n, p, jitter=0.15, seed=seed)
     X = m.node_coords.reshape(-1, 3)
-    out = dict(workload="closure3d", steps=steps,
-               describe=f"3D APE, {n}^3x6 jittered Kuhn tets (K={m.K}), p=3, mixed precision, boundary faces "
-                        "absorbing or learned")
-    nfp = 10
-    params = dg.to_bf16(dg.closure_init(nfp, seed=seed, out_scale=0.1, dim=3))
-    zero = dg.to_bf16(dg.closure_init(nfp, seed=seed, out_scale=0.0, dim=3))
+    shape = f"{n}^3x6 jittered Kuhn tets" if dim == 3 else f"{n}^2x2 jittered triangles"
+    out = dict(workload=f"closure{dim}d", steps=steps,
+               describe=f"{dim}D APE, {shape} (K={m.K}), p={p}, mixed precision, boundary faces absorbing or learned")
+    nfp = dg.num_nodes(dim - 1, p) if dim == 3 else p + 1
+    params = dg.to_bf16(dg.closure_init(nfp, seed=seed, out_scale=0.1, dim=dim))
+    zero = dg.to_bf16(dg.closure_init(nfp, seed=seed, out_scale=0.0, dim=dim))
+    sides = 0x3F if dim == 3 else 0xF
     for mode, code, net in (("characteristic", dg.DG_BC_ABSORBING, None), ("learned_zero", dg.DG_BC_LEARNED, zero),
                             ("learned", dg.DG_BC_LEARNED, params)):
-        bc = dg.resolve_bc_sides(m, [(0x3F, code)])
-        s = dg.Solver(m.node_coords, m.element_vertices, 3, bc, v0=(30.0, 0.0, 0.0), rho0=RHO0, c0=340.0,
+        bc = dg.resolve_bc_sides(m, [(sides, code)])
+        s = dg.Solver(m.node_coords, m.element_vertices, p, bc, v0=(30.0, 0.0, 0.0), rho0=RHO0, c0=340.0,
                       precision=dg.DG_MIXED, steps_per_graph=20)
         if net is not None:
             s.set_closure(net)
         s.add_monopole(0.0, 0.0, 0.0, 0.05, 0.1, 200.0, 0.0, 1e9)
-        for k in range(8):  # Fibonacci points on r = 0.8
-            z = 1.0 - (2 * k + 1) / 8.0
-            r, ph = np.sqrt(1.0 - z * z), np.pi * (3.0 - np.sqrt(5.0)) * k
-            s.add_observer(0.8 * r * np.cos(ph), 0.8 * r * np.sin(ph), 0.8 * z, 0.03)
+        for k in range(8):  # Fibonacci points on r = 0.8 (3D) / equally spaced on the circle (2D)
+            if dim == 3:
+                z = 1.0 - (2 * k + 1) / 8.0
+                r, ph = np.sqrt(1.0 - z * z), np.pi * (3.0 - np.sqrt(5.0)) * k
+                s.add_observer(0.8 * r * np.cos(ph), 0.8 * r * np.sin(ph), 0.8 * z, 0.03)
+            else:
+                s.add_observer(0.8 * np.cos(np.pi * k / 4), 0.8 * np.sin(np.pi * k / 4), 0.0, 0.03)
         info = s.info()
-        dt = info.min_inradius / ((340.0 + 30.0) * 7)
+        dt = info.min_inradius / ((340.0 + 30.0) * (2 * p + 1))
         s.set_state(np.zeros((4, len(X))))
         s.run_steps(warmup, dt)
         s.set_state(np.zeros((4, len(X))))
@@ -138,10 +145,13 @@ def run(name, steps, warmup=10):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["closure3d"], default="dense3d")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["closure2d", "closure3d"], default="dense3d")
     ap.add_argument("--steps", type=int, default=200)
     a = ap.parse_args(argv)
-    res = run_closure3d(a.steps) if a.workload == "closure3d" else run(a.workload, a.steps)
+    if a.workload.startswith("closure"):
+        res = run_closure(int(a.workload[7]), a.steps)
+    else:
+        res = run(a.workload, a.steps)
     print(json.dumps(res), flush=True)
 
 
