@@ -167,28 +167,47 @@ dgk::StageArgs<T> stageArgs(dg_context* c, int stage, const void* qin, void* qou
     return a;
 }
 
-template <int DIM, int P, int NF, typename T, bool COMP>
+template <int DIM, int P, int NF, typename T, bool COMP, bool WIDE>
 void launchStage(dg_context* c, int stage, const void* qin, void* qout, void* uout, bool snap) {
     using dgk::stageKernel;
     const dgk::StageArgs<T> a = stageArgs<T, COMP>(c, stage, qin, qout, uout, snap);
-    const dim3 grid(static_cast<unsigned>(c->nblocks)), block(dgk::StageWaves<DIM, P, NF, T>::kBlockSize);
+    const dim3 grid(static_cast<unsigned>(c->nblocks)), block(dgk::StageWaves<DIM, P, NF, T, WIDE>::kBlockSize);
     switch (dgk::stageMode(stage)) {
         case dgk::kModeFirst:
-            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeFirst>), grid, block, 0, c->stream, a);
+            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeFirst, WIDE>), grid, block, 0, c->stream, a);
             break;
         case dgk::kModeMid:
-            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeMid>), grid, block, 0, c->stream, a);
+            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeMid, WIDE>), grid, block, 0, c->stream, a);
             break;
         case dgk::kModeFinal:
-            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeFinal>), grid, block, 0, c->stream, a);
+            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeFinal, WIDE>), grid, block, 0, c->stream, a);
             break;
         case dgk::kModeEuler:
-            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeEuler>), grid, block, 0, c->stream, a);
+            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeEuler, WIDE>), grid, block, 0, c->stream, a);
             break;
         default:
-            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeRHS>), grid, block, 0, c->stream, a);
+            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeRHS, WIDE>), grid, block, 0, c->stream, a);
             break;
     }
+}
+
+// 512-thread stage blocks need a grid of at least this many of them (4 per CU); smaller grids
+// run 256-thread blocks, which spread the same work over twice the blocks
+constexpr int64_t kWideMinBlocks = 1024;
+
+template <int DIM, int P, int NF, typename T, bool COMP>
+void pickStage(dg_context* c) {
+    using Wd = dgk::StageWaves<DIM, P, NF, T, true>;
+    using Nw = dgk::StageWaves<DIM, P, NF, T, false>;
+    if constexpr (Wd::kBig) {
+        if ((c->K + Wd::EB - 1) / Wd::EB < kWideMinBlocks) {
+            c->eb = Nw::EB;
+            c->stageFn = &launchStage<DIM, P, NF, T, COMP, false>;
+            return;
+        }
+    }
+    c->eb = Wd::EB;
+    c->stageFn = &launchStage<DIM, P, NF, T, COMP, true>;
 }
 
 dgk::SourceArgs sourceArgs(dg_context* c) {
@@ -409,13 +428,11 @@ bool pick(dg_context* c) {
     }
     c->mfma = false;
     if (c->precision == DG_FP64) {
-        c->eb = dgk::StageWaves<DIM, P, NF, double>::EB;
-        c->stageFn = &launchStage<DIM, P, NF, double, false>;
+        pickStage<DIM, P, NF, double, false>(c);
         c->sourceFn = &launchSource<double, false>;
         c->postFn = &launchPost<double, false>;
     } else {
-        c->eb = dgk::StageWaves<DIM, P, NF, float>::EB;
-        c->stageFn = &launchStage<DIM, P, NF, float, true>;
+        pickStage<DIM, P, NF, float, true>(c);
         c->sourceFn = &launchSource<float, true>;
         c->postFn = &launchPost<float, true>;
     }

@@ -259,13 +259,14 @@ __device__ inline void blockResiduals(const double (&res)[5], double (*sRed)[5],
     }
 }
 
-template <int DIM, int P, int NF, typename T>
+template <int DIM, int P, int NF, typename T, bool WIDE = true>
 struct StageWaves {
     static constexpr int NP = Sizes<DIM, P>::NP;
-    static constexpr bool kBig = NF == 4 && NP >= 10 && NP <= 20;
+    static constexpr bool kBig = WIDE && NF == 4 && NP >= 10 && NP <= 20;
     // 4 fields, 10..20 nodes: 512-thread blocks (one operator table per 2x the elements) fit
     // 4 blocks = 8 waves per SIMD in LDS in FP32, 2 blocks = 4 waves per SIMD in FP64; the
-    // kernel is asked to stay within the registers that allows
+    // kernel is asked to stay within the registers that allows. WIDE = false keeps 256-thread
+    // blocks, for grids too small to give every CU several 512-thread blocks
     static constexpr int kBlockSize = kBig ? 512 : kBlock;
     static constexpr int value = kBig ? (sizeof(T) == 4 ? 8 : 4) : 1;
     static constexpr int EB = kBlockSize / NP;  // elements per block
@@ -292,12 +293,12 @@ __host__ __device__ constexpr int stageMode(int stage) {
 // batch, the element's own stage input, inverse Jacobians, face rows, neighbour codes, and the
 // RK operands consumed at the end) -> LDS -> barrier; phase 2 (the neighbour traces of all of the
 // thread's face nodes) -> fluxes -> barrier; then the contractions and the RK update.
-template <int DIM, int P, int NF, typename T, bool COMP, int MODE>
-__global__ __launch_bounds__((StageWaves<DIM, P, NF, T>::kBlockSize), (StageWaves<DIM, P, NF, T>::value)) void stageKernel(StageArgs<T> a) {
+template <int DIM, int P, int NF, typename T, bool COMP, int MODE, bool WIDE = true>
+__global__ __launch_bounds__((StageWaves<DIM, P, NF, T, WIDE>::kBlockSize), (StageWaves<DIM, P, NF, T, WIDE>::value)) void stageKernel(StageArgs<T> a) {
     using S = Sizes<DIM, P>;
     using L = OpLayout<DIM, P>;
-    constexpr int kBlock = StageWaves<DIM, P, NF, T>::kBlockSize;  // threads per block
-    constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC, NFT = S::NFT, EB = StageWaves<DIM, P, NF, T>::EB;
+    constexpr int kBlock = StageWaves<DIM, P, NF, T, WIDE>::kBlockSize;  // threads per block
+    constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC, NFT = S::NFT, EB = StageWaves<DIM, P, NF, T, WIDE>::EB;
     constexpr int JJ = DIM * DIM;
     constexpr int TS = static_cast<int>(sizeof(T));
     constexpr bool kOpsInLds = (L::kTotal * TS) <= 40 * 1024;
