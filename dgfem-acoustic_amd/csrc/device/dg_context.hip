@@ -74,6 +74,8 @@ struct dg_context {
     int* nonfinite = nullptr;
     int64_t* counter = nullptr;
     int64_t* batchSteps = nullptr;
+    int64_t* srcBase = nullptr;  // steps of the batch done by earlier captured sequences
+    bool obsSrcOverlap = false;  // some observer node is also a source node
     int64_t ringSlots = 0;
     // pinned staging
     double *hT = nullptr, *hV = nullptr, *hRes = nullptr, *hObs = nullptr;
@@ -82,6 +84,10 @@ struct dg_context {
     // graphs
     hipGraphExec_t gMulti = nullptr, gOne = nullptr;
     int gSteps = 0, gIntegrator = -1, gResid = -1;
+    hipStream_t side = nullptr;  // branch of the end-of-step kernel in captured RK4 sequences
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
+    hipStream_t postStream = nullptr;  // stream of the next launchPost (null: the context stream)
+    bool postSources = true;           // launchPost also overwrites the next step's sources
     // dispatch
     StageFn stageFn = nullptr;
     int eb = 1;              // elements per block (batch) of the chosen stage kernel
@@ -313,12 +319,20 @@ void launchSource(dg_context* c) {
 }
 
 template <typename T, bool COMP>
+void launchSourceStep(dg_context* c, int64_t offset) {
+    const unsigned nb = static_cast<unsigned>((c->nSrcNodes + dgk::kSrcU * 256 - 1) / (dgk::kSrcU * 256));
+    hipLaunchKernelGGL((dgk::sourceStepKernel<T, COMP>), dim3(nb), dim3(256), 0, c->stream, sourceArgs(c), c->srcBase,
+                       offset, c->batchSteps, static_cast<T*>(c->u), static_cast<T*>(COMP ? c->ulo : nullptr));
+}
+
+template <typename T, bool COMP>
 void launchPost(dg_context* c) {
+    hipStream_t st = c->postStream ? c->postStream : c->stream;
     dgk::PostArgs p{};
     p.resPartial = c->resPartial;
     p.nblocks = c->nblocks;
     if (c->resGroup && c->curResid) {
-        hipLaunchKernelGGL(dgk::residualGroupKernel, dim3(static_cast<unsigned>(c->nResGroups)), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL(dgk::residualGroupKernel, dim3(static_cast<unsigned>(c->nResGroups)), dim3(256), 0, st,
                            static_cast<const double*>(c->resPartial), c->nblocks, c->resGroup);
         p.resPartial = c->resGroup;
         p.nblocks = c->nResGroups;
@@ -337,7 +351,8 @@ void launchPost(dg_context* c) {
     p.stepCounter = c->counter;
     p.batchSteps = c->batchSteps;
     p.computeResiduals = c->curResid;
-    hipLaunchKernelGGL((dgk::postKernel<T, COMP>), dim3(1), dim3(dgk::kPostBlock), 0, c->stream, p, sourceArgs(c),
+    p.withSources = c->postSources ? 1 : 0;
+    hipLaunchKernelGGL((dgk::postKernel<T, COMP>), dim3(1), dim3(dgk::kPostBlock), 0, st, p, sourceArgs(c),
                        static_cast<T*>(c->u), static_cast<T*>(COMP ? c->ulo : nullptr));
 }
 
@@ -541,6 +556,10 @@ void syncTables(dg_context* c) {
     if (!cover.empty()) coverOff.push_back(static_cast<int64_t>(coverSrc.size()));
     for (size_t i = 0; i < srcNodes.size(); ++i) lastSrc.push_back(coverSrc[coverOff[i + 1] - 1]);
     c->nSrcNodes = static_cast<int64_t>(srcNodes.size());
+    c->obsSrcOverlap = false;
+    for (const std::vector<int64_t>& nd : c->obsNodes)
+        for (int64_t n : nd)
+            if (std::binary_search(srcNodes.begin(), srcNodes.end(), n)) c->obsSrcOverlap = true;
     // entry 0 of every table is valid memory for the clamped loads of the kernels
     if (srcNodes.empty()) {
         srcNodes.push_back(0);
@@ -618,12 +637,47 @@ void launchStep(dg_context* c, int integrator, bool resid) {
     c->postFn(c);
 }
 
+// Step `s` of a captured RK4 sequence with the end-of-step kernel (residuals, observers, step
+// counter) on a branch of its own. The next step's first three stages neither write u/ulo nor
+// touch the residual partials, so only its final stage waits for the branch; the next step's
+// source overwrite runs on the main branch (sourceStepKernel). Used when no observer node is a
+// source node, so that the observers' reads and the overwrite touch different nodes.
+void launchStepForked(dg_context* c, bool resid, int s) {
+    runStage(c, dgk::kStageRK1, c->u, c->qa, c->u, false);
+    runStage(c, dgk::kStageRK2, c->qa, c->qb, c->u, false);
+    runStage(c, dgk::kStageRK3, c->qb, c->qa, c->u, false);
+    if (s > 0) HIPCHK(hipStreamWaitEvent(c->stream, c->evJoin, 0));
+    runStage(c, dgk::kStageRK4, c->qa, nullptr, c->u, resid);
+    HIPCHK(hipEventRecord(c->evFork, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->side, c->evFork, 0));
+    c->postStream = c->side;
+    c->postSources = false;
+    c->postFn(c);
+    c->postStream = nullptr;
+    c->postSources = true;
+    HIPCHK(hipEventRecord(c->evJoin, c->side));
+    if (c->nSrcNodes > 0) {
+        if (c->precision == DG_FP64)
+            launchSourceStep<double, false>(c, s + 1);
+        else
+            launchSourceStep<float, true>(c, s + 1);
+    }
+}
+
 void captureGraphs(dg_context* c, int integrator, bool resid, int steps) {
     releaseGraphs(c);
+    const bool fork = integrator == DG_RK4 && !c->obsSrcOverlap;
     auto capture = [&](int n) {
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (int s = 0; s < n; ++s) launchStep(c, integrator, resid);
+        if (fork) {
+            for (int s = 0; s < n; ++s) launchStepForked(c, resid, s);
+            HIPCHK(hipStreamWaitEvent(c->stream, c->evJoin, 0));
+            if (c->nSrcNodes > 0)
+                hipLaunchKernelGGL(dgk::advanceKernel, dim3(1), dim3(1), 0, c->stream, c->srcBase, static_cast<int64_t>(n));
+        } else {
+            for (int s = 0; s < n; ++s) launchStep(c, integrator, resid);
+        }
         HIPCHK(hipStreamEndCapture(c->stream, &g));
         hipGraphExec_t ex;
         HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
@@ -709,12 +763,16 @@ void destroyContext(dg_context* c) {
     devFree(c->nonfinite);
     devFree(c->counter);
     devFree(c->batchSteps);
+    devFree(c->srcBase);
     devFree(c->dLearned);
     devFree(c->ghost);
     devFree(c->dClosureW);
     for (void* h : {static_cast<void*>(c->hT), static_cast<void*>(c->hV), static_cast<void*>(c->hRes),
                     static_cast<void*>(c->hObs), static_cast<void*>(c->hVF)})
         if (h) (void)hipHostFree(h);
+    if (c->evFork) (void)hipEventDestroy(c->evFork);
+    if (c->evJoin) (void)hipEventDestroy(c->evJoin);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -784,6 +842,9 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         c->device = c->opt.device;
         HIPCHK(hipSetDevice(c->device));
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming));
         c->nblocks = (c->K + c->eb - 1) / c->eb;  // one batch of eb elements per block
         const size_t nstate = static_cast<size_t>(c->K) * c->Np * c->nf;
         auto alloc = [&](size_t n) -> void* {
@@ -857,6 +918,7 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         c->nonfinite = devAlloc<int>(c, 1);
         c->counter = devAlloc<int64_t>(c, 1);
         c->batchSteps = devAlloc<int64_t>(c, 1);
+        c->srcBase = devAlloc<int64_t>(c, 1);
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hT), sizeof(double) * c->ringSlots));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hRes), sizeof(double) * c->ringSlots * 5));
         c->tablesDirty = true;
@@ -1109,7 +1171,7 @@ dg_status dg_run(dg_context* c, const dg_run_params* prm, dg_step_fn on_step, dg
                 HIPCHK(hipMemcpyAsync(c->dVF, c->hVF, n * ns, hipMemcpyHostToDevice, c->stream));
             }
             hipLaunchKernelGGL(dgk::setCounterKernel, dim3(1), dim3(1), 0, c->stream, c->counter, c->batchSteps,
-                               n, c->nonfinite);
+                               n, c->nonfinite, c->srcBase);
             HIPCHK(hipEventRecord(ev0, c->stream));
             c->sourceFn(c);
             if (useGraph) {
@@ -1240,7 +1302,7 @@ dg_status dg_profile_steps(dg_context* c, double dt, int32_t steps, double* stag
         for (size_t s = 0; s < tt.size(); ++s) c->hT[s] = s * dt;
         HIPCHK(hipMemcpyAsync(c->dT, c->hT, sizeof(double) * tt.size(), hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(dgk::setCounterKernel, dim3(1), dim3(1), 0, c->stream, c->counter, c->batchSteps,
-                           static_cast<int64_t>(tt.size()), c->nonfinite);
+                           static_cast<int64_t>(tt.size()), c->nonfinite, c->srcBase);
         std::vector<hipEvent_t> ev(14);
         for (auto& e : ev) HIPCHK(hipEventCreate(&e));
         double acc[6] = {0, 0, 0, 0, 0, 0};

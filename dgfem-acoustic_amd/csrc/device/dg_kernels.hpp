@@ -1459,6 +1459,7 @@ struct PostArgs {
     int64_t* stepCounter;
     const int64_t* batchSteps;  // number of steps in the current batch
     int computeResiduals;
+    int withSources;            // 0: the next step's source overwrite runs in sourceStepKernel
 };
 
 // End-of-step work in one block of kPostBlock threads: residuals (src/solver.cpp:518-547;
@@ -1495,7 +1496,7 @@ __global__ __launch_bounds__(kPostBlock) void postKernel(PostArgs p, SourceArgs 
     const int oc = tid < p.nobs ? tid : 0;
     const int sg0 = p.obsSeg0[oc], sg1 = p.obsSeg0[oc + 1];
     SourceRound sr;
-    const bool haveSrc = s.nnodes > 0;
+    const bool haveSrc = p.withSources && s.nnodes > 0;
     if (haveSrc) loadSourceRound(s, tid, sr);
     // residual partial sums of this thread: stage blocks tid, tid + kPostBlock, ... in order,
     // U per round trip
@@ -1604,10 +1605,33 @@ __global__ __launch_bounds__(kPostBlock) void postKernel(PostArgs p, SourceArgs 
     if (tid == 0) *p.stepCounter = slot + 1;
 }
 
-[[maybe_unused]] static __global__ void setCounterKernel(int64_t* counter, int64_t* batchSteps, int64_t nsteps, int* flag) {
+[[maybe_unused]] static __global__ void setCounterKernel(int64_t* counter, int64_t* batchSteps, int64_t nsteps, int* flag,
+                                                         int64_t* srcBase) {
     *counter = 0;
     *batchSteps = nsteps;
     *flag = -1;
+    *srcBase = 0;
 }
+
+// Source overwrite of step `*base + offset` of the batch (when that step exists) inside a captured
+// sequence of steps, where the end-of-step kernel runs on a branch of its own: one round of kSrcU
+// table entries per thread, as many 256-thread blocks as the table needs. `*base` is the number of
+// steps of the batch that earlier sequences did (advanceKernel adds to it after each sequence).
+template <typename T, bool COMP>
+__global__ __launch_bounds__(256) void sourceStepKernel(SourceArgs s, const int64_t* __restrict__ base, int64_t offset,
+                                                        const int64_t* __restrict__ batchSteps, T* __restrict__ u,
+                                                        T* __restrict__ ulo) {
+    const int64_t slot = *base + offset;
+    const int64_t nsteps = *batchSteps;
+    SourceRound r;
+    loadSourceRound(s, static_cast<int64_t>(blockIdx.x) * kSrcU * blockDim.x + threadIdx.x, r);
+    if (slot >= nsteps) return;
+    bool act[kSrcU];
+    double v[kSrcU];
+    evalSourceRound(s, slot, s.tTable[slot], r, act, v);
+    writeSourceRound<T, COMP>(s, r, act, v, u, ulo);
+}
+
+[[maybe_unused]] static __global__ void advanceKernel(int64_t* base, int64_t n) { *base += n; }
 
 }  // namespace dgk

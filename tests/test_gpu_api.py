@@ -179,3 +179,36 @@ def test_residuals_of_a_large_grid_match_the_host_sum():
     np.testing.assert_allclose(r.residuals[-1], host, rtol=1e-10, atol=0)
     assert min(host) > 0
     s.close()
+
+
+@pytest.mark.parametrize("overlap", [False, True])
+@pytest.mark.parametrize("precision", [dg.DG_FP64, dg.DG_MIXED])
+def test_captured_steps_match_eager_steps(precision, overlap):
+    """Captured RK4 sequences run the end-of-step kernel on a branch of its own and the next
+    step's source overwrite on the main branch when no observer node is a source node (else the
+    single end-of-step kernel, as the eager path always does). Both must give the eager results
+    bit for bit: state, residuals and observers over batches of several sequence lengths, with a
+    monopole that switches off mid-run and a table source."""
+    print(f"case captured vs eager precision={precision} overlap={overlap}", flush=True)
+    m, bc, _ = _case()
+    u = gaussian_state(m, -0.3, 0.2, 0.0, 0.05)
+    out = []
+    for use_graph in (False, True):
+        s = dg.Solver(m.node_coords, m.element_vertices, 3, bc, v0=V0, rho0=RHO0, c0=C0, precision=precision,
+                      use_graph=use_graph, steps_per_graph=4)
+        s.add_monopole(0.1, -0.1, 0.0, 0.3, 0.5, 900.0, 0.2, 31 * 2e-5)
+        s.add_table_source(-0.5, -0.5, 0.0, 0.25, lambda t: 0.2 * np.cos(3000.0 * t) if t < 40 * 2e-5 else None)
+        s.add_observer(0.6, 0.5, 0.0, 0.3)
+        s.add_observer(-0.6, 0.4, 0.0, 0.2)
+        if overlap:
+            s.add_observer(0.1, -0.1, 0.0, 0.2)
+        s.set_state(u)
+        r = s.run(0.0, 52.5 * 2e-5, 2e-5, 9 * 2e-5)
+        assert r.steps == 53
+        out.append((s.get_state(), r.residuals, r.observers))
+        s.close()
+    (u0, r0, o0), (u1, r1, o1) = out
+    np.testing.assert_array_equal(u1, u0)
+    np.testing.assert_array_equal(r1, r0)
+    np.testing.assert_array_equal(o1, o0)
+    assert np.abs(o0[:, 0]).max() > 0
