@@ -6,7 +6,9 @@
  *   (include/solver.h:25, src/solver.cpp:311-584)
  * and its sibling solver::forwardEuler (include/solver.h:16, src/solver.cpp:66-302).
  * Plain pointers and sizes only; status codes instead of exit(); one context per host
- * thread; the context owns its device memory, stream and graphs.
+ * thread; the context owns its device memory, its graphs and two streams: the main one, and
+ * one for the end-of-step kernel's graph branch inside captured RK4 steps, which every
+ * captured sequence joins back into the main stream before it ends.
  *
  * State layout at this boundary is the reference's: u[field][element * Np + node] with
  * fields (p, vx, vy, vz) — plus rho for the 5-field variant — and the element's node order
