@@ -183,13 +183,15 @@ def test_residuals_of_a_large_grid_match_the_host_sum():
 
 @pytest.mark.parametrize("overlap", [False, True])
 @pytest.mark.parametrize("precision", [dg.DG_FP64, dg.DG_MIXED])
-def test_captured_steps_match_eager_steps(precision, overlap):
-    """Captured RK4 sequences run the end-of-step kernel on a branch of its own and the next
+def test_captured_steps_match_eager_steps(precision, overlap, monkeypatch):
+    """Captured RK4 sequences may run the end-of-step kernel on a branch of its own and the next
     step's source overwrite on the main branch when no observer node is a source node (else the
-    single end-of-step kernel, as the eager path always does). Both must give the eager results
-    bit for bit: state, residuals and observers over batches of several sequence lengths, with a
-    monopole that switches off mid-run and a table source."""
+    single end-of-step kernel, as the eager path always does); DGFEM_POST_BRANCH=1 asks for the
+    branch on this small grid. Both must give the eager results bit for bit: state, residuals
+    and observers over batches of several sequence lengths, with a monopole that switches off
+    mid-run and a table source."""
     print(f"case captured vs eager precision={precision} overlap={overlap}", flush=True)
+    monkeypatch.setenv("DGFEM_POST_BRANCH", "1")
     m, bc, _ = _case()
     u = gaussian_state(m, -0.3, 0.2, 0.0, 0.05)
     out = []
