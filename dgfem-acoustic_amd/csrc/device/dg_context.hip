@@ -664,10 +664,11 @@ void launchStepForked(dg_context* c, bool resid, int s) {
     }
 }
 
-// The branch costs a cross-stream dependency each way per step (about 10 µs per step on MI355X):
-// it pays on large stage grids, whose end-of-step kernel reads many residual partials, and not
-// on small ones (a 2D grid of 1,536 blocks ran 39% slower per step with it). DGFEM_POST_BRANCH
-// = 0 | 1 overrides the rule.
+// The branch adds a cross-stream dependency each way per step. On MI355X a 7,865-block 3D grid
+// with a source and two observers breaks even with it (the end-of-step kernel's ~19 µs hidden,
+// about as much added), and a 1,536-block 2D grid ran 39% slower per step. It is kept for large
+// grids, where the end-of-step kernel grows with the residual partials and the observer and
+// source tables while the branch's cost does not. DGFEM_POST_BRANCH = 0 | 1 overrides the rule.
 constexpr int64_t kBranchMinBlocks = 4096;
 
 bool usePostBranch(const dg_context* c, int integrator) {
