@@ -24,6 +24,11 @@ namespace {
 
 thread_local std::string g_createError;
 
+// A stage-kernel variant that was asked for but is not built for the mesh.
+struct UnsupportedError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
 struct TableSource {
     dg_value_fn fn;
     void* user;
@@ -75,7 +80,6 @@ struct dg_context {
     int64_t* counter = nullptr;
     int64_t* batchSteps = nullptr;
     int64_t* srcBase = nullptr;  // steps of the batch done by earlier captured sequences
-    bool obsSrcOverlap = false;  // some observer node is also a source node
     int64_t ringSlots = 0;
     // pinned staging
     double *hT = nullptr, *hV = nullptr, *hRes = nullptr, *hObs = nullptr;
@@ -92,6 +96,7 @@ struct dg_context {
     StageFn stageFn = nullptr;
     int eb = 1;              // elements per block (batch) of the chosen stage kernel
     bool mfma = false;       // matrix-core stage kernel in use
+    int kernelKind = DG_KERNEL_VALU;  // stage kernel chosen by dg_create (reported by dg_get_info)
     void* afrag = nullptr;   // A-operand fragments of the matrix-core kernel
     std::vector<double> afragHost;
     PerStepFn sourceFn = nullptr, postFn = nullptr;
@@ -166,6 +171,7 @@ dgk::StageArgs<T> stageArgs(dg_context* c, int stage, const void* qin, void* qou
     a.iops = c->iops;
     a.snap = snap ? static_cast<const T*>(c->snap) : nullptr;
     a.resPartial = c->resPartial;
+    a.stateBad = c->nonfinite + 1;
     a.K = c->K;
     a.dt = static_cast<T>(c->curDt);
     a.stage = stage;
@@ -400,16 +406,26 @@ void pickAux(dg_context* c) {
     }
 }
 
+// Stage-kernel variant: DGFEM_KERNEL = valu (default) | mm (block matrix-core, 4 fields)
+// | wave (wave-autonomous matrix-core, 4 fields). An explicit request for a variant that is not
+// built for the mesh's (dim, order, fields) fails dg_create with DG_ERR_UNSUPPORTED.
 template <int DIM, int P, int NF>
 bool pick(dg_context* c) {
     pickAux<DIM, P, NF>(c);
-    // Stage-kernel variant: DGFEM_KERNEL = valu (default) | mm (block matrix-core, 4 fields)
-    // | wave (wave-autonomous matrix-core, 4 fields).
     const char* env = std::getenv("DGFEM_KERNEL");
-    const std::string kind = env ? env : "valu";
+    const std::string kind = env && env[0] ? env : "valu";
+    if (kind != "valu" && kind != "mm" && kind != "wave")
+        throw UnsupportedError("DGFEM_KERNEL=" + kind + " is not a stage-kernel variant (valu, mm, wave)");
+    if (kind == "mm" && !(NF == 4 && dgk::BlockMmShape<DIM, P>::kSupported))
+        throw UnsupportedError("DGFEM_KERNEL=mm is built for 4 fields on tetrahedra of order <= 3 and triangles "
+                               "of order <= 5 only");
+    if (kind == "wave" && !(NF == 4 && dgk::MfmaShape<DIM, P>::kSupported))
+        throw UnsupportedError("DGFEM_KERNEL=wave is built for 4 fields on tetrahedra of order <= 3 and "
+                               "triangles of order <= 5 only");
     if constexpr (NF == 4 && dgk::BlockMmShape<DIM, P>::kSupported) {
         if (kind == "mm") {
             c->mfma = true;
+            c->kernelKind = DG_KERNEL_MM;
             buildAfragLane<DIM, P>(c, dg::refElement(DIM, P));
             c->eb = dgk::BlockMmShape<DIM, P>::EB;
             if (c->precision == DG_FP64) {
@@ -427,6 +443,7 @@ bool pick(dg_context* c) {
     if constexpr (NF == 4 && dgk::MfmaShape<DIM, P>::kSupported) {
         if (kind == "wave") {
             c->mfma = true;
+            c->kernelKind = DG_KERNEL_WAVE;
             buildAfrag<DIM, P>(c, dg::refElement(DIM, P));
             c->eb = dgk::WaveShape<DIM, P>::EB;
             if (c->precision == DG_FP64) {
@@ -442,6 +459,7 @@ bool pick(dg_context* c) {
         }
     }
     c->mfma = false;
+    c->kernelKind = DG_KERNEL_VALU;
     if (c->precision == DG_FP64) {
         pickStage<DIM, P, NF, double, false>(c);
         c->sourceFn = &launchSource<double, false>;
@@ -556,10 +574,6 @@ void syncTables(dg_context* c) {
     if (!cover.empty()) coverOff.push_back(static_cast<int64_t>(coverSrc.size()));
     for (size_t i = 0; i < srcNodes.size(); ++i) lastSrc.push_back(coverSrc[coverOff[i + 1] - 1]);
     c->nSrcNodes = static_cast<int64_t>(srcNodes.size());
-    c->obsSrcOverlap = false;
-    for (const std::vector<int64_t>& nd : c->obsNodes)
-        for (int64_t n : nd)
-            if (std::binary_search(srcNodes.begin(), srcNodes.end(), n)) c->obsSrcOverlap = true;
     // entry 0 of every table is valid memory for the clamped loads of the kernels
     if (srcNodes.empty()) {
         srcNodes.push_back(0);
@@ -671,8 +685,20 @@ void launchStepForked(dg_context* c, bool resid, int s) {
 // source tables while the branch's cost does not. DGFEM_POST_BRANCH = 0 | 1 overrides the rule.
 constexpr int64_t kBranchMinBlocks = 4096;
 
+// Whether some observer node is also a source node, from the host definitions (the same test
+// syncTables makes when it builds the device tables).
+bool observersOverlapSources(const dg_context* c) {
+    std::vector<int64_t> src;
+    for (const std::vector<int64_t>& nd : c->srcNodes) src.insert(src.end(), nd.begin(), nd.end());
+    std::sort(src.begin(), src.end());
+    for (const std::vector<int64_t>& nd : c->obsNodes)
+        for (int64_t n : nd)
+            if (std::binary_search(src.begin(), src.end(), n)) return true;
+    return false;
+}
+
 bool usePostBranch(const dg_context* c, int integrator) {
-    if (integrator != DG_RK4 || c->obsSrcOverlap) return false;
+    if (integrator != DG_RK4 || observersOverlapSources(c)) return false;
     const char* env = std::getenv("DGFEM_POST_BRANCH");
     if (env && env[0] == '0') return false;
     if (env && env[0] == '1') return true;
@@ -843,7 +869,15 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
                             "of order <= 3";
             return DG_ERR_UNSUPPORTED;
         }
-        if (!pickKernels(c)) {
+        bool picked = false;
+        try {
+            picked = pickKernels(c);
+        } catch (const UnsupportedError& ex) {
+            delete c;
+            g_createError = std::string("dg_create: ") + ex.what();
+            return DG_ERR_UNSUPPORTED;
+        }
+        if (!picked) {
             delete c;
             g_createError = "dg_create: no kernel built for this (dim, order)";
             return DG_ERR_UNSUPPORTED;
@@ -930,7 +964,7 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         }
         c->ringSlots = 4096;
         c->resRing = devAlloc<double>(c, c->ringSlots * 5);
-        c->nonfinite = devAlloc<int>(c, 1);
+        c->nonfinite = devAlloc<int>(c, 2);  // first non-finite slot, state flag of the final stages
         c->counter = devAlloc<int64_t>(c, 1);
         c->batchSteps = devAlloc<int64_t>(c, 1);
         c->srcBase = devAlloc<int64_t>(c, 1);
@@ -1210,7 +1244,7 @@ dg_status dg_run(dg_context* c, const dg_run_params* prm, dg_step_fn on_step, dg
             HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
             devMs += ms;
             int64_t lim = n;
-            if (flag >= 0 && resid) lim = flag + 1;
+            if (flag >= 0) lim = flag + 1;
             for (int64_t s = 0; s < lim; ++s) {
                 if (on_step) {
                     dg_step_record rec{done + s, times[done + s], &c->hRes[s * 5], no, no ? &c->hObs[s * no * 4] : nullptr,
@@ -1223,13 +1257,13 @@ dg_status dg_run(dg_context* c, const dg_run_params* prm, dg_step_fn on_step, dg
                     }
                 }
             }
-            if (flag >= 0 && resid) {
+            if (flag >= 0) {
                 (void)hipEventDestroy(ev0);
                 (void)hipEventDestroy(ev1);
                 if (steps_done) *steps_done = done + flag + 1;
                 c->lastRunMs = devMs;
                 return fail(c, DG_ERR_NONFINITE,
-                            "dg_run: non-finite residual at step " + std::to_string(done + flag));
+                            "dg_run: non-finite residual or state at step " + std::to_string(done + flag));
             }
             done = end;
         }
@@ -1302,6 +1336,10 @@ dg_status dg_get_info(const dg_context* c, dg_info* info) {
     for (double v : c->mesh.inradius) r = std::min(r, v);
     info->min_inradius = r;
     info->device_bytes = c->deviceBytes;
+    info->stage_kernel = c->kernelKind;
+    info->post_branch = (c->opt.use_graph != 0 && usePostBranch(c, DG_RK4)) ? 1 : 0;
+    info->elements_per_block = c->eb;
+    info->stage_blocks = c->nblocks;
     return DG_OK;
 }
 

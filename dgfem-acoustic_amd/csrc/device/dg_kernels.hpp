@@ -73,6 +73,7 @@ struct StageArgs {
     const int32_t* __restrict__ iops;
     const T* __restrict__ snap;       // residual snapshot (leading word of the state), RK4/Euler final stage
     double* __restrict__ resPartial;  // [gridDim][5]
+    int* __restrict__ stateBad;       // set to 1 by a final stage that produced a non-finite state value
     int64_t K;
     T dt;
     int stage;
@@ -208,6 +209,16 @@ __device__ inline void storeNode(T* __restrict__ p, const T (&v)[NF]) {
 #pragma unroll
         for (int k = 0; k < NF; ++k) p[k] = v[k];
     }
+}
+
+// Whether every value of a node's new state is finite (a non-finite state is reported by the
+// end-of-step kernel whether or not residuals are computed).
+template <int NF, typename T>
+__device__ inline bool allFinite(const T (&v)[NF]) {
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < NF; ++k) ok = ok && __builtin_isfinite(v[k]);
+    return ok;
 }
 
 template <typename T>
@@ -590,6 +601,7 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T, WIDE>::kBlockSize), (Sta
             }
             storeNodeNT<NF>(a.uout + base, unew);
             if constexpr (COMP) storeNodeNT<NF>(a.ulo + base, lnew);
+            if (!allFinite<NF>(unew)) a.stateBad[0] = 1;
             if (a.snap) {
                 double d[4];
                 double sv[4];
@@ -898,6 +910,7 @@ __global__ __launch_bounds__(kBlock) void stageKernelWave(
                 V4 oh;
                 oh.x = hi[0]; oh.y = hi[1]; oh.z = hi[2]; oh.w = hi[3];
                 reinterpret_cast<V4*>(a.uout)[off] = oh;
+                if (!allFinite<4>(hi)) a.stateBad[0] = 1;
                 if constexpr (COMP) {
                     V4 ol;
                     ol.x = lw[0]; ol.y = lw[1]; ol.z = lw[2]; ol.w = lw[3];
@@ -1276,6 +1289,7 @@ __global__ __launch_bounds__(kBlock) void stageKernelMM(StageArgs<T> a, const T*
             }
             storeNodeNT<NF>(a.uout + base, unew);
             if constexpr (COMP) storeNodeNT<NF>(a.ulo + base, lw);
+            if (!allFinite<NF>(unew)) a.stateBad[0] = 1;
             if (a.snap) {
                 double sv[4], d[4];
 #pragma unroll
@@ -1455,7 +1469,8 @@ struct PostArgs {
     int nf;
     double* resRing;            // [slots][5]
     double* obsRing;            // [slots][nobs][4]
-    int* nonfinite;             // first slot with a non-finite residual, or -1
+    int* nonfinite;             // [0]: first slot with a non-finite residual or state, or -1;
+                                // [1]: set by a final stage that wrote a non-finite state value
     int64_t* stepCounter;
     const int64_t* batchSteps;  // number of steps in the current batch
     int computeResiduals;
@@ -1483,6 +1498,7 @@ __global__ __launch_bounds__(kPostBlock) void postKernel(PostArgs p, SourceArgs 
     // ---- round trip 1 ----
     const int64_t slot = *p.stepCounter;
     const int64_t nsteps = *p.batchSteps;
+    const int stateBad = p.nonfinite[1];
     int64_t oIdx[4];
     double oW[4];
     {
@@ -1582,6 +1598,7 @@ __global__ __launch_bounds__(kPostBlock) void postKernel(PostArgs p, SourceArgs 
         p.resRing[slot * 5 + tid] = v;
         if (!isfinite(v)) atomicCAS(p.nonfinite, -1, static_cast<int>(slot));
     }
+    if (tid == 0 && stateBad) atomicCAS(p.nonfinite, -1, static_cast<int>(slot));
     // observers: sum of the observer's segment partials in segment order
     for (int o = tid; o < p.nobs; o += kPostBlock) {
         const int s0 = o == tid ? sg0 : p.obsSeg0[o], s1 = o == tid ? sg1 : p.obsSeg0[o + 1];
@@ -1609,7 +1626,8 @@ __global__ __launch_bounds__(kPostBlock) void postKernel(PostArgs p, SourceArgs 
                                                          int64_t* srcBase) {
     *counter = 0;
     *batchSteps = nsteps;
-    *flag = -1;
+    flag[0] = -1;
+    flag[1] = 0;
     *srcBase = 0;
 }
 

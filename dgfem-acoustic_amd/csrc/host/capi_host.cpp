@@ -102,6 +102,9 @@ dg_status dg_mesh_precompute(const dg_mesh_desc* m, double* Jinv, int32_t* nbr, 
 #include "formula.hpp"
 #include "sources_io.hpp"
 
+// Meshes with at most this many boundary faces list each of them in the case summary.
+constexpr int64_t kSummaryMaxBoundaryFaces = 4096;
+
 extern "C" {
 
 double dg_eval_formula(const char* expression, double t) {
@@ -166,6 +169,19 @@ dg_status dg_case_summary(const char* config_path, char* out, int64_t capacity) 
         for (size_t i = 0; i < c.observers.size(); ++i)
             js << (i ? "," : "") << "[" << c.observers[i][0] << "," << c.observers[i][1] << "," << c.observers[i][2] << ","
                << c.observers[i][3] << "]";
+        js << "],\"boundary_faces\":[";
+        // per boundary face (element in file order, Gmsh local face, type), small meshes only
+        int64_t nB = 0;
+        for (size_t i = 0; i < isB.size(); ++i) nB += isB[i] ? 1 : 0;
+        if (nB <= kSummaryMaxBoundaryFaces) {
+            bool firstFace = true;
+            for (size_t i = 0; i < isB.size(); ++i) {
+                if (!isB[i]) continue;
+                js << (firstFace ? "" : ",") << "[" << i / (cs.mesh.dim + 1) << "," << i % (cs.mesh.dim + 1) << ","
+                   << static_cast<int>(bc[i]) << "]";
+                firstFace = false;
+            }
+        }
         js << "],\"initial_conditions\":[";
         for (size_t i = 0; i < c.initConditions.size(); ++i) {
             js << (i ? "," : "") << "[";

@@ -23,6 +23,8 @@ DG_FP64, DG_MIXED = 0, 1
 DG_FIELDS_REF4, DG_FIELDS_LEE5 = 0, 1
 DG_BC_ABSORBING, DG_BC_REFLECTING, DG_BC_LEARNED = 0, 1, 2
 DG_RK4, DG_EULER = 0, 1
+DG_KERNEL_VALU, DG_KERNEL_MM, DG_KERNEL_WAVE = 0, 1, 2
+KERNEL_NAMES = {DG_KERNEL_VALU: "valu", DG_KERNEL_MM: "mm", DG_KERNEL_WAVE: "wave"}
 
 
 class DGError(RuntimeError):
@@ -65,7 +67,8 @@ class Info(C.Structure):
     _fields_ = [("dim", C.c_int32), ("order", C.c_int32), ("nodes_per_element", C.c_int32),
                 ("nodes_per_face", C.c_int32), ("fields", C.c_int32), ("precision", C.c_int32),
                 ("num_elements", C.c_int64), ("num_faces", C.c_int64), ("num_boundary_faces", C.c_int64),
-                ("min_inradius", C.c_double), ("device_bytes", C.c_int64)]
+                ("min_inradius", C.c_double), ("device_bytes", C.c_int64), ("stage_kernel", C.c_int32),
+                ("post_branch", C.c_int32), ("elements_per_block", C.c_int32), ("stage_blocks", C.c_int64)]
 
 
 VALUE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.POINTER(C.c_double))

@@ -31,7 +31,7 @@ enum {
     DG_ERR_INVALID = 1,     /* bad argument */
     DG_ERR_MESH = 2,        /* malformed / unsupported mesh */
     DG_ERR_DEVICE = 3,      /* HIP runtime error or no device */
-    DG_ERR_NONFINITE = 4,   /* non-finite residual detected during dg_run */
+    DG_ERR_NONFINITE = 4,   /* non-finite state or residual detected during dg_run */
     DG_ERR_UNSUPPORTED = 5, /* order / dimension / option not built */
     DG_ERR_CALLBACK = 6     /* a caller callback returned nonzero */
 };
@@ -42,6 +42,10 @@ enum { DG_FIELDS_REF4 = 0, DG_FIELDS_LEE5 = 1 };    /* (p,vx,vy,vz) or + rho */
    face takes its exterior state from the learned closure network (dg_set_closure). */
 enum { DG_BC_ABSORBING = 0, DG_BC_REFLECTING = 1, DG_BC_LEARNED = 2 };
 enum { DG_RK4 = 0, DG_EULER = 1 };                  /* config timeIntMethod */
+/* Stage-kernel variants (dg_info.stage_kernel): the VALU kernel (default), the block and the
+   wave-autonomous FP32/FP64 matrix-core kernels (environment DGFEM_KERNEL = valu | mm | wave;
+   an explicit request that is not built for the mesh fails dg_create with DG_ERR_UNSUPPORTED). */
+enum { DG_KERNEL_VALU = 0, DG_KERNEL_MM = 1, DG_KERNEL_WAVE = 2 };
 
 typedef struct dg_context dg_context;
 
@@ -124,8 +128,10 @@ typedef int (*dg_save_fn)(void* user, int64_t step, double t, const double* stat
 /* solver::rungeKutta / solver::forwardEuler time loop. Callbacks may be NULL. Steps run on the
    device in batches (at most steps_per_graph steps, never across a save step); the step records
    of a batch are delivered after it. When a step callback returns nonzero (DG_ERR_CALLBACK) or a
-   residual is non-finite (DG_ERR_NONFINITE), *steps_done counts the steps delivered up to and
-   including that step, and the state held by the context is the one at the end of its batch. */
+   step leaves a non-finite value in the state or in a residual (DG_ERR_NONFINITE; the state is
+   checked whether or not residuals are computed), *steps_done counts the steps delivered up to
+   and including that step, and the state held by the context is the one at the end of its
+   batch. */
 DG_API dg_status dg_run(dg_context* ctx, const dg_run_params* prm, dg_step_fn on_step, dg_save_fn on_save,
                         void* user, int64_t* steps_done);
 
@@ -142,11 +148,20 @@ typedef struct {
     int64_t num_elements, num_faces, num_boundary_faces;
     double min_inradius;
     int64_t device_bytes;
+    int32_t stage_kernel;        /* DG_KERNEL_* chosen by dg_create */
+    int32_t post_branch;         /* 1: captured RK4 steps run the end-of-step kernel on a graph branch
+                                    of its own (stage grids of >= 4096 blocks, no observer node that
+                                    is also a source node, graphs on) */
+    int32_t elements_per_block;  /* elements per stage-kernel block */
+    int64_t stage_blocks;        /* blocks per stage launch */
 } dg_info;
 DG_API dg_status dg_get_info(const dg_context* ctx, dg_info* info);
 
 /* Per-kernel device time of `steps` eager RK4 steps measured with HIP events on the
-   context's stream: stage_ms[4] (mean per launch of stages 1..4), source_ms, post_ms. */
+   context's stream: stage_ms[4] (mean per launch of stages 1..4), source_ms, post_ms.
+   The steps are real: they advance the context's state by `steps` RK4 steps, with the
+   source clock of those steps restarting at t = 0 and residuals against the current
+   snapshot. */
 DG_API dg_status dg_profile_steps(dg_context* ctx, double dt, int32_t steps, double* stage_ms, double* source_ms,
                                   double* post_ms);
 
@@ -239,7 +254,9 @@ DG_API dg_status dg_read_source_table(const char* path, double* rows, int64_t ca
 /* Parse a .conf/.json configuration and its mesh (src/configParser.cpp:33-450), resolve the
    boundary types, and write a JSON summary into out[capacity] (keys: mesh, dim, order,
    elements, nodes_per_element, time, integrator, mean_flow, bc_counts, sources, observers,
-   initial_conditions, physical_bcs). */
+   initial_conditions, physical_bcs, boundary_faces: [element, Gmsh local face, 0 absorbing |
+   1 reflecting] per boundary face, listed for meshes of at most 4096 boundary faces, else
+   empty). */
 DG_API dg_status dg_case_summary(const char* config_path, char* out, int64_t capacity);
 
 /* Diagnostics of the host precompute (no device needed), for tests of the host logic.

@@ -562,6 +562,14 @@ int oracle_num_volume_points(const oracle_ctx* c) { return c->G; }
 int oracle_num_face_points(const oracle_ctx* c) { return c->Gf; }
 int64_t oracle_num_faces(const oracle_ctx* c) { return c->F; }
 
+/* Boundary type of the local face lf of element e (src/Mesh.cpp:579-605): 0 absorbing,
+   1 reflecting, -1 interior. */
+int oracle_face_bc(const oracle_ctx* c, int64_t e, int lf) {
+    if (e < 0 || e >= c->K || lf < 0 || lf >= c->Nfc) return -2;
+    const int64_t f = c->elF[e * c->Nfc + lf];
+    return c->fBnd[f] ? c->fBC[f] : -1;
+}
+
 /* ------------------------------------------------------------ RHS operators ---- */
 /* Mesh::updateFlux (src/Mesh.cpp:851-975): nodal physical fluxes and boundary ghosts. */
 static void update_flux(oracle_ctx* c, const double* u) {

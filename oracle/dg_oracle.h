@@ -104,6 +104,8 @@ void oracle_bc_flux(const oracle_physics* ph, int bc, const double n[3], const d
 int oracle_num_volume_points(const oracle_ctx* ctx);
 int oracle_num_face_points(const oracle_ctx* ctx);
 int64_t oracle_num_faces(const oracle_ctx* ctx);
+/* Boundary type of local face lf of element e: 0 absorbing, 1 reflecting, -1 interior. */
+int oracle_face_bc(const oracle_ctx* ctx, int64_t e, int lf);
 
 #ifdef __cplusplus
 }

@@ -72,6 +72,7 @@ def load(build_if_missing: bool = True) -> C.CDLL:
     lib.oracle_num_face_points.argtypes = [C.c_void_p]
     lib.oracle_num_faces.argtypes = [C.c_void_p]
     lib.oracle_num_faces.restype = C.c_int64
+    lib.oracle_face_bc.argtypes = [C.c_void_p, C.c_int64, C.c_int]
     lib.oracle_bc_flux.argtypes = [P(OPhysics), C.c_int] + [P(C.c_double)] * 5
     _lib = lib
     return lib
@@ -146,6 +147,11 @@ class Oracle:
     @property
     def num_faces(self):
         return self._lib.oracle_num_faces(self._h)
+
+    def face_bc(self):
+        """Boundary types [K][dim+1] per local face: 0 absorbing, 1 reflecting, -1 interior."""
+        K, nfc = self.K, self.dim + 1
+        return np.array([[self._lib.oracle_face_bc(self._h, e, f) for f in range(nfc)] for e in range(K)])
 
     def rhs(self, u, dt, threads=0):
         u = np.ascontiguousarray(u, dtype=np.float64).reshape(4, self.K * self.Np)

@@ -204,6 +204,7 @@ def test_captured_steps_match_eager_steps(precision, overlap, monkeypatch):
         s.add_observer(-0.6, 0.4, 0.0, 0.2)
         if overlap:
             s.add_observer(0.1, -0.1, 0.0, 0.2)
+        assert s.info().post_branch == (1 if use_graph and not overlap else 0)
         s.set_state(u)
         r = s.run(0.0, 52.5 * 2e-5, 2e-5, 9 * 2e-5)
         assert r.steps == 53
@@ -214,3 +215,44 @@ def test_captured_steps_match_eager_steps(precision, overlap, monkeypatch):
     np.testing.assert_array_equal(r1, r0)
     np.testing.assert_array_equal(o1, o0)
     assert np.abs(o0[:, 0]).max() > 0
+
+
+@pytest.fixture(scope="module")
+def large_grid():
+    # 41^3 x 6 p=3 tets: 16,541 stage blocks of 25 elements (more than 4 x 4,096, so the residual
+    # partials are folded in groups, and more than 4,096, so the default rule takes the branch)
+    m = dg.box_mesh(3, 41, 3, jitter=0.15, seed=21)
+    bc = dg.resolve_bc_sides(m, [(0x3F, dg.DG_BC_ABSORBING)])
+    return m, bc
+
+
+@pytest.mark.parametrize("precision", [dg.DG_FP64, dg.DG_MIXED])
+def test_captured_steps_match_eager_on_a_large_grid(large_grid, precision, monkeypatch):
+    """The default branch rule on a grid of more than 16,384 stage blocks: captured sequences run
+    the end-of-step kernel on its branch (asserted through dg_get_info) with a monopole and two
+    observers, grouped residual sums and a save step inside the run; state, residuals and
+    observers equal the eager run's bit for bit."""
+    print(f"case captured vs eager large grid precision={precision}", flush=True)
+    monkeypatch.delenv("DGFEM_POST_BRANCH", raising=False)
+    m, bc = large_grid
+    u = gaussian_state(m, 0.1, -0.2, 0.05, 0.02)
+    out = []
+    for use_graph in (False, True):
+        s = dg.Solver(m.node_coords, m.element_vertices, 3, bc, v0=(30.0, 0.0, 0.0), rho0=RHO0, c0=340.0,
+                      precision=precision, use_graph=use_graph, steps_per_graph=4)
+        s.add_monopole(0.0, 0.0, 0.0, 0.1, 0.1, 2000.0, 0.0, 1.0)
+        s.add_observer(0.5, 0.0, 0.0, 0.1)
+        s.add_observer(0.0, -0.5, 0.25, 0.1)
+        info = s.info()
+        assert info.stage_blocks > 16384 and info.elements_per_block == 25
+        assert info.post_branch == (1 if use_graph else 0)
+        s.set_state(u)
+        r = s.run(0.0, 10.5 * 2e-6, 2e-6, 6 * 2e-6)
+        assert r.steps == 11
+        out.append((s.get_state(), r.residuals, r.observers))
+        s.close()
+    (u0, r0, o0), (u1, r1, o1) = out
+    np.testing.assert_array_equal(u1, u0)
+    np.testing.assert_array_equal(r1, r0)
+    np.testing.assert_array_equal(o1, o0)
+    assert np.abs(o0[:, :, 0]).max() > 0 and r0[:, 0].min() > 0

@@ -1,0 +1,105 @@
+"""The high-order dense case: 3D LEE with the passive density, p=5 (56 nodes), 5 fields.
+
+Configured size: [-1,1]^3 as 44^3 hexes x 6 jittered Kuhn tets (511,104 tets, 143M DOF), a
+seeded sum of 8 Gaussian pulses, reflecting at z=-1 and absorbing elsewhere (physical groups in
+ascending tag order: absorbing, then reflecting), CFL-limited dt, 2000 RK4 steps: mixed
+precision within 1e-6 of FP64 in pressure (field and observer series), and the density stays
+p/c0^2 (the isentropic invariant of the 5-field variant).
+
+At reduced size the 5-field variant's (p, v) must equal the 4-field oracle (the reference
+algorithm restated) to 1e-10: the density is passive and does not feed back.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import dgfem_amd as dg
+from cases import make_case, oracle_for, rel_l2
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "dgfem-acoustic_amd"))
+import workloads  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+RHO0 = 1.225
+C0 = 340.0
+
+
+def _pulses(m, seed, nf=5):
+    rng = np.random.default_rng(seed)
+    X = m.node_coords.reshape(-1, 3)
+    p = np.zeros(len(X))
+    for _ in range(8):
+        c = rng.uniform(-0.6, 0.6, 3)
+        w = rng.uniform(0.05, 0.15)
+        p += rng.normal(0.0, 1.0) * np.exp(-((X - c) ** 2).sum(axis=1) / (w * w))
+    u = np.zeros((nf, len(X)))
+    u[0] = p
+    if nf == 5:
+        u[4] = p / (C0 * C0)
+    return u
+
+
+@pytest.mark.parametrize("p", [3, 5])
+def test_lee5_pressure_and_velocity_equal_the_four_field_oracle(p):
+    """3^3 x 6 tets, reflecting z=-1 + absorbing, 8 pulses, v0 = (30, -10, 5), 100 RK4 steps in
+    FP64: fields (p, vx, vy, vz) of the 5-field run against the 4-field oracle, rho = p/c0^2."""
+    print(f"case lee5 vs 4-field oracle p={p}", flush=True)
+    groups = [(0x2F, dg.DG_BC_ABSORBING), (0x10, dg.DG_BC_REFLECTING)]
+    v0 = (30.0, -10.0, 5.0)
+    m, bc, gn = make_case(3, 3, p, groups, jitter=0.15, seed=3)
+    u5 = _pulses(m, seed=11)
+    s = dg.Solver(m.node_coords, m.element_vertices, p, bc, v0=v0, rho0=RHO0, c0=C0, fields=dg.DG_FIELDS_LEE5)
+    dt = s.info().min_inradius / ((C0 + np.linalg.norm(v0)) * (2 * p + 1))
+    s.set_state(u5)
+    s.add_observer(0.2, -0.3, -0.8, 0.3)
+    r = s.run_steps(100, dt)
+    u = s.get_state()
+    o = oracle_for(m, gn, groups, v0, RHO0, C0)
+    ref = o.run(u5[:4], 0.0, 99.5 * dt, dt, 1e9, observers=[(0.2, -0.3, -0.8, 0.3)])
+    assert ref["steps"] == 100
+    for f in range(4):
+        e = rel_l2(u[f], ref["u"][f])
+        print(f"field {f}: rel L2 {e:.2e}")
+        assert e < 1e-10
+    np.testing.assert_allclose(r.observers, ref["observers"], rtol=1e-8, atol=1e-14)
+    # the 4-field residuals are those of the oracle (rho's residual is p's scaled by 1/c0^4)
+    np.testing.assert_allclose(r.residuals[:, [0, 2, 3, 4]], ref["residuals"][:, [0, 2, 3, 4]], rtol=1e-8,
+                               atol=1e-300)
+    assert rel_l2(u[4], u[0] / C0 ** 2) < 1e-12
+    assert rel_l2(u[0], u5[0]) > 1e-3
+
+
+@pytest.mark.timeout(900)
+def test_dense_case_2000_steps_mixed_within_1e6_of_fp64():
+    print("case dense 44^3x6 p5 lee5 2000 steps mixed vs fp64", flush=True)
+    w = workloads.dense3d()
+    m = w["mesh"]
+    assert m.K == 511104 and m.Np == 56
+    obs = [(0.5, 0.0, 0.0, 0.05), (0.0, -0.5, 0.25, 0.05), (-0.3, 0.3, -0.9, 0.05)]
+    out = {}
+    for prec in (dg.DG_FP64, dg.DG_MIXED):
+        s = dg.Solver(m.node_coords, m.element_vertices, 5, w["bc"], v0=w["v0"], rho0=RHO0, c0=w["c0"],
+                      fields=w["fields"], precision=prec, steps_per_graph=20)
+        dt = s.info().min_inradius / ((w["c0"] + np.linalg.norm(w["v0"])) * 11)
+        for ob in obs:
+            assert s.add_observer(*ob) > 0
+        s.set_state(w["u0"])
+        r = s.run_steps(2000, dt)
+        assert r.steps == 2000
+        u = s.get_state()
+        out[prec] = (u[0].copy(), r.observers[:, :, 0].copy(), rel_l2(u[4], u[0] / C0 ** 2),
+                     [rel_l2(u[f], w["u0"][f]) for f in (0,)])
+        print(f"precision {prec}: {r.device_ms / 2000:.3f} ms/step, rho invariant {out[prec][2]:.2e}", flush=True)
+        del u
+        s.close()
+    p64, o64, inv64, _ = out[dg.DG_FP64]
+    pmx, omx, invmx, _ = out[dg.DG_MIXED]
+    e = rel_l2(pmx, p64)
+    eo = rel_l2(omx, o64)
+    print(f"dense case mixed vs fp64 after 2000 steps: pressure {e:.2e}, observers {eo:.2e}")
+    assert inv64 < 1e-12 and invmx < 1e-6
+    assert out[dg.DG_FP64][3][0] > 1e-2  # the field has evolved
+    assert e <= 1e-6 and eo <= 1e-6

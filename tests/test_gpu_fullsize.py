@@ -148,3 +148,43 @@ def test_c2_full_size_two_rk4_steps_match_oracle(c2):
         assert rel_l2(u[f], ref["u"][f]) < 1e-11
     np.testing.assert_allclose(r.residuals[:, :4], ref["residuals"][:, :4], rtol=1e-9, atol=1e-30)
     np.testing.assert_allclose(r.observers, ref["observers"], rtol=1e-10, atol=1e-14)
+
+
+def _c2_solver(m, bc, precision):
+    s = dg.Solver(m.node_coords, m.element_vertices, 3, bc, v0=(30.0, 0.0, 0.0), rho0=RHO0, c0=340.0,
+                  precision=precision, steps_per_graph=20)
+    assert s.add_monopole(0.0, 0.0, 0.0, 0.1, 0.1, 200.0, 0.0, 1e9) > 0
+    assert s.add_observer(0.5, 0.0, 0.0, 0.1) > 0
+    assert s.add_observer(0.0, -0.5, 0.25, 0.1) > 0
+    return s
+
+
+@pytest.mark.timeout(600)
+def test_c2_2000_steps_mixed_within_1e6_of_fp64(c2):
+    """The headline condition at the configured size and length: the 3D APE case (196,608 tets,
+    p=3, monopole at the origin, absorbing, CFL-limited dt) from a zero state, 2000 RK4 steps in
+    FP64 and in mixed precision. Pressure field and observer pressure series: rel L2 <= 1e-6."""
+    print("case c2 full 32^3x6 p3 2000 steps mixed vs fp64", flush=True)
+    m, bc, _, _ = c2
+    probe = _c2_solver(m, bc, dg.DG_FP64)
+    dt = probe.info().min_inradius / ((340.0 + 30.0) * 7)
+    probe.close()
+    out = {}
+    for prec in (dg.DG_FP64, dg.DG_MIXED):
+        s = _c2_solver(m, bc, prec)
+        r = s.run_steps(2000, dt)
+        assert r.steps == 2000
+        out[prec] = (s.get_state(), r.observers)
+        s.close()
+        print(f"precision {prec}: {r.device_ms / 2000:.4f} ms/step", flush=True)
+    u64, o64 = out[dg.DG_FP64]
+    umx, omx = out[dg.DG_MIXED]
+    e = [rel_l2(umx[f], u64[f]) for f in range(4)]
+    eo = rel_l2(omx[:, :, 0], o64[:, :, 0])
+    print(f"rel L2 mixed vs fp64 after 2000 steps: fields {['%.2e' % x for x in e]}, observers p {eo:.2e}")
+    # the wave has filled the box: the pressure is nonzero at the far corner region
+    X = m.node_coords.reshape(-1, 3)
+    far = np.linalg.norm(X, axis=1) > 0.9
+    assert np.abs(u64[0][far]).max() > 1e-3 * np.abs(u64[0]).max()
+    assert e[0] <= 1e-6 and eo <= 1e-6
+    assert max(e[1:]) <= 1e-5

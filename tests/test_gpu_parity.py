@@ -17,6 +17,11 @@ pytestmark = pytest.mark.gpu
 RHO0 = 1.225
 
 
+def matrix_core_built(dim, p):
+    """Shapes with FP32/FP64 matrix-core stage kernels: triangles, tetrahedra of order <= 3."""
+    return dim == 2 or p <= 3
+
+
 @pytest.mark.parametrize("dim,n,p,groups,v0,c0", [
     (2, 4, 1, [(ALL2, 1)], (-30, 30, 0), 100.0),
     (2, 4, 2, [(0x3, 1), (0xC, 0)], (-30, 30, 0), 100.0),
@@ -31,11 +36,19 @@ RHO0 = 1.225
 ])
 @pytest.mark.parametrize("kernel", ["valu", "mm", "wave"])
 def test_rhs_matches_oracle(dim, n, p, groups, v0, c0, kernel, monkeypatch):
+    """The matrix-core variants are built for triangles and for tetrahedra of order <= 3: on
+    other shapes an explicit request must fail dg_create with DG_ERR_UNSUPPORTED."""
     print(f"case rhs dim={dim} n={n} p={p} kernel={kernel}", flush=True)
     monkeypatch.setenv("DGFEM_KERNEL", kernel)
     m, bc, gn = make_case(dim, n, p, groups, seed=7)
+    if kernel != "valu" and not matrix_core_built(dim, p):
+        with pytest.raises(dg.DGError) as ei:
+            dg.Solver(m.node_coords, m.element_vertices, p, bc, v0=v0, rho0=RHO0, c0=c0)
+        assert ei.value.code == dg.DG_ERR_UNSUPPORTED and f"DGFEM_KERNEL={kernel}" in str(ei.value)
+        return
     o = oracle_for(m, gn, groups, v0, RHO0, c0)
     s = dg.Solver(m.node_coords, m.element_vertices, p, bc, v0=v0, rho0=RHO0, c0=c0)
+    assert dg.KERNEL_NAMES[s.info().stage_kernel] == kernel
     u = smooth_state(m)
     k_ref = o.rhs(u, 1e-4)
     k_gpu = s.rhs(u, 1e-4)
@@ -88,6 +101,7 @@ def test_rk4_3d_absorbing_monopole_matches_oracle(kernel, monkeypatch):
     u0 = np.zeros((4, m.K * m.Np))
     ref = o.run(u0, 0.0, (steps - 0.5) * dt, dt, 1e9, sources=[src], observers=obs)
     s = dg.Solver(m.node_coords, m.element_vertices, 3, bc, v0=v0, rho0=RHO0, c0=340.0)
+    assert dg.KERNEL_NAMES[s.info().stage_kernel] == kernel
     s.set_state(u0)
     assert s.add_monopole(0, 0, 0, 0.3, 0.1, 200.0, 0.0, 1.0) > 0
     s.add_observer(*obs[0])
@@ -196,6 +210,7 @@ def test_euler_3d_with_sources_matches_oracle(kernel, monkeypatch):
     for prec in (dg.DG_FP64, dg.DG_MIXED):
         s = dg.Solver(m.node_coords, m.element_vertices, 3, bc, v0=v0, rho0=RHO0, c0=340.0, precision=prec,
                       steps_per_graph=7)
+        assert dg.KERNEL_NAMES[s.info().stage_kernel] == kernel
         s.set_state(u0)
         assert s.add_monopole(-0.2, 0.1, 0.0, 0.3, 0.05, 400.0, 0.3, 1.0) > 0
         s.add_observer(*obs[0])
@@ -239,6 +254,7 @@ def test_mixed_precision_close_to_fp64(kernel, monkeypatch):
     res = []
     for prec in (dg.DG_FP64, dg.DG_MIXED):
         s = dg.Solver(m.node_coords, m.element_vertices, 3, bc, v0=(30, 0, 0), rho0=RHO0, c0=340.0, precision=prec)
+        assert dg.KERNEL_NAMES[s.info().stage_kernel] == kernel
         s.set_state(gaussian_state(m, size=0.02))
         s.add_monopole(0, 0, 0, 0.2, 0.1, 200.0, 0.0, 1.0)
         s.add_observer(0.5, 0.3, 0.0, 0.2)
@@ -300,12 +316,35 @@ def test_lee5_keeps_isentropic_density():
     assert np.linalg.norm(u[0] - u0[0]) > 0
 
 
-def test_nonfinite_is_reported():
-    print("case nonfinite", flush=True)
+@pytest.mark.parametrize("residuals", [True, False])
+@pytest.mark.parametrize("precision", [dg.DG_FP64, dg.DG_MIXED])
+def test_nonfinite_is_reported(residuals, precision):
+    """A run that blows up stops with DG_ERR_NONFINITE whether or not residuals are on."""
+    print(f"case nonfinite residuals={residuals} precision={precision}", flush=True)
     groups = [(ALL2, 1)]
     m, bc, gn = make_case(2, 4, 2, groups)
-    s = dg.Solver(m.node_coords, m.element_vertices, 2, bc, v0=(0, 0, 0), rho0=RHO0, c0=100.0)
+    s = dg.Solver(m.node_coords, m.element_vertices, 2, bc, v0=(0, 0, 0), rho0=RHO0, c0=100.0,
+                  precision=precision, steps_per_graph=5)
     s.set_state(gaussian_state(m, size=0.1))
     with pytest.raises(dg.DGError) as ei:
-        s.run(0.0, 1e9, 10.0, 1e30, max_steps=2000)
+        s.run(0.0, 1e9, 10.0, 1e30, max_steps=2000, residuals=residuals)
     assert ei.value.code == dg.DG_ERR_NONFINITE
+    # the failing step is the first one whose state is not finite
+    n = ei.value.result.steps
+    assert 1 <= n < 2000
+
+
+@pytest.mark.parametrize("residuals", [True, False])
+def test_nonfinite_initial_state_is_reported(residuals):
+    """A non-finite value put into the state by the host is reported at the first step."""
+    print(f"case nonfinite initial state residuals={residuals}", flush=True)
+    groups = [(ALL2, 0)]
+    m, bc, gn = make_case(2, 4, 2, groups)
+    s = dg.Solver(m.node_coords, m.element_vertices, 2, bc, v0=(0, 0, 0), rho0=RHO0, c0=100.0)
+    u = gaussian_state(m, size=0.1)
+    u[1, 17] = np.nan
+    s.set_state(u)
+    with pytest.raises(dg.DGError) as ei:
+        s.run(0.0, 9.5e-6, 1e-6, 1e30, residuals=residuals)
+    assert ei.value.code == dg.DG_ERR_NONFINITE
+    assert ei.value.result.steps == 1
