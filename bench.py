@@ -152,6 +152,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=25.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--mixed-kernel", default=None,
+                    help="stage kernel of the mixed path (DGFEM_KERNEL value; default: the library's choice)")
     ap.add_argument("--accuracy-steps", type=int, default=2000,
                     help="steps of the mixed-vs-FP64 accuracy leg (the metric's condition is at 2000)")
     args = ap.parse_args()
@@ -176,7 +178,11 @@ def main():
     states = {}
     kernels = {}
     for name, prec in (("fp64", dg.DG_FP64), ("mixed", dg.DG_MIXED)):
+        if name == "mixed" and args.mixed_kernel:
+            os.environ["DGFEM_KERNEL"] = args.mixed_kernel
         s = make_solver(m, bc, prec, args.steps_per_graph)
+        if args.mixed_kernel:
+            os.environ.pop("DGFEM_KERNEL", None)
         kernels[name] = dg.KERNEL_NAMES[s.info().stage_kernel]
         states[name], r, wall = accuracy_then_timed_run(s, args.accuracy_steps, args.steps, args.warmup, dt)
         ms = s.profile_steps(dt, args.profile_steps)
@@ -200,7 +206,9 @@ def main():
     mixed = out["mixed"]
     kernel_name = {"valu": "stageKernel<3,3,4,float,true>",
                    "wave": "stageKernelWave<3,3,float,true>",
-                   "mm": "stageKernelMM<3,3,float,true>"}[kernels["mixed"]]
+                   "mm": "stageKernelMM<3,3,float,true>",
+                   "bf16x6": "stageKernelBf16<3,3,4,3>",
+                   "bf16x3": "stageKernelBf16<3,3,4,2>"}[kernels["mixed"]]
     stage_sum_ms = sum(mixed["stage_ms"])
     achieved = BYTES_PER_DOF["mixed"] * dof / (stage_sum_ms * 1e-3) / 1e9
     achieved_wall = BYTES_PER_DOF["mixed"] * dof / (mixed["ms_per_step"] * 1e-3) / 1e9

@@ -19,10 +19,12 @@
 #include "../host/refelem.hpp"
 #include "dg_closure.hpp"
 #include "dg_kernels.hpp"
+#include "dg_kernels_bf16.hpp"
 
 namespace {
 
 thread_local std::string g_createError;
+
 
 // A stage-kernel variant that was asked for but is not built for the mesh.
 struct UnsupportedError : std::runtime_error {
@@ -98,6 +100,11 @@ struct dg_context {
     bool mfma = false;       // matrix-core stage kernel in use
     int kernelKind = DG_KERNEL_VALU;  // stage kernel chosen by dg_create (reported by dg_get_info)
     void* afrag = nullptr;   // A-operand fragments of the matrix-core kernel
+    void* afragBf = nullptr;  // split-BF16 operator fragments (uint16) of the BF16 matrix-core kernel
+    std::vector<uint16_t> afragBfHost;
+    int64_t bfGroups = 0;     // 16-element groups of the BF16 matrix-core kernel (persistent grid), or 0
+    int bfWaves = 0;          // its waves per block
+    int bfSplit = 0;          // BF16 terms per operand (2 or 3)
     std::vector<double> afragHost;
     PerStepFn sourceFn = nullptr, postFn = nullptr;
     int64_t deviceBytes = 0;
@@ -273,6 +280,68 @@ void launchStageMM(dg_context* c, int stage, const void* qin, void* qout, void* 
     }
 }
 
+// Split-BF16 matrix-core stage (instances in dg_stage_bf16.hip).
+void launchStageBf16(dg_context* c, int stage, const void* qin, void* qout, void* uout, bool snap) {
+    const dgk::StageArgs<float> a = stageArgs<float, true>(c, stage, qin, qout, uout, snap);
+    if (!dgk::launchStageBf16(c->dim, c->p, c->nf, c->bfSplit, dgk::stageMode(stage), a, c->afragBf, c->bfGroups,
+                              static_cast<unsigned>(c->nblocks), c->stream))
+        throw std::runtime_error("split-BF16 stage kernel not built for this shape");
+}
+
+// Nearest BF16 (bit pattern) of a double, through the nearest float.
+uint16_t toBf16(double x) {
+    const float f = static_cast<float>(x);
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+
+double fromBf16(uint16_t b) {
+    const uint32_t u = static_cast<uint32_t>(b) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// Operator fragments of the BF16 matrix-core kernel, [MT][KS][NSPLIT][64 lanes][8]: lane l of
+// (row tile it, k-step ks) holds row 16 it + l % 16 and the 8 columns of chunk 4 ks + l / 16 of
+// [Dw_1 | .. | Dw_DIM | -LIFT] in the kernel's chunk order (volume chunks (r, c): Dw_r columns
+// 8c .. 8c+7; lift chunks c: -LIFT columns 8c .. 8c+7; zero outside). Each FP64 value is split
+// into NSPLIT BF16 terms, each the nearest BF16 of what the previous terms leave.
+template <int DIM, int P, int NSPLIT>
+void buildAfragBf16(dg_context* c, const dg::RefElement& R) {
+    using B = dgk::Bf16Shape<DIM, P>;
+    const int Np = R.Np, NFT = R.Nfaces * R.Nfp;
+    c->afragBfHost.assign(static_cast<size_t>(B::MT) * B::KS * NSPLIT * 64 * 8, 0);
+    for (int it = 0; it < B::MT; ++it)
+        for (int ks = 0; ks < B::KS; ++ks)
+            for (int lane = 0; lane < 64; ++lane) {
+                const int row = 16 * it + lane % 16, g = lane / 16;
+                for (int jj = 0; jj < 8; ++jj) {
+                    double v = 0.0;
+                    if (row < Np) {
+                        if (ks < B::KSV) {
+                            const int ch = 4 * ks + g;
+                            if (ch < DIM * B::CPR) {
+                                const int r = ch / B::CPR, j = 8 * (ch % B::CPR) + jj;
+                                if (j < Np) v = R.Dw[(static_cast<size_t>(r) * Np + row) * Np + j];
+                            }
+                        } else {
+                            const int fa = 8 * (4 * (ks - B::KSV) + g) + jj;
+                            if (fa < NFT) v = -R.LIFT[static_cast<size_t>(row) * NFT + fa];
+                        }
+                    }
+                    double rem = v;
+                    for (int s = 0; s < NSPLIT; ++s) {
+                        const uint16_t t = toBf16(rem);
+                        rem -= fromBf16(t);
+                        c->afragBfHost[((((static_cast<size_t>(it) * B::KS + ks) * NSPLIT + s) * 64 + lane) * 8) + jj] = t;
+                    }
+                }
+            }
+}
+
 template <int DIM, int P>
 void buildAfrag(dg_context* c, const dg::RefElement& R);
 
@@ -406,16 +475,48 @@ void pickAux(dg_context* c) {
     }
 }
 
+template <int DIM, int P, int NF, int NSPLIT>
+void pickBf16(dg_context* c) {
+    c->mfma = false;
+    c->kernelKind = NSPLIT == 3 ? DG_KERNEL_BF16X6 : DG_KERNEL_BF16X3;
+    buildAfragBf16<DIM, P, NSPLIT>(c, dg::refElement(DIM, P));
+    c->eb = dgk::Bf16Shape<DIM, P>::kGroup;
+    c->bfGroups = (c->K + c->eb - 1) / c->eb;
+    c->bfWaves = dgk::Bf16Shape<DIM, P>::kWaves;
+    c->bfSplit = NSPLIT;
+    c->stageFn = &launchStageBf16;
+    c->sourceFn = &launchSource<float, true>;
+    c->postFn = &launchPost<float, true>;
+}
+
 // Stage-kernel variant: DGFEM_KERNEL = valu (default) | mm (block matrix-core, 4 fields)
-// | wave (wave-autonomous matrix-core, 4 fields). An explicit request for a variant that is not
-// built for the mesh's (dim, order, fields) fails dg_create with DG_ERR_UNSUPPORTED.
+// | wave (wave-autonomous matrix-core, 4 fields) | bf16x6, bf16x3 (split-BF16 matrix-core,
+// mixed precision; 3 and 2 BF16 terms per operand). An explicit request for a variant that is
+// not built for the mesh's (dim, order, fields, precision) fails dg_create with
+// DG_ERR_UNSUPPORTED.
 template <int DIM, int P, int NF>
 bool pick(dg_context* c) {
     pickAux<DIM, P, NF>(c);
     const char* env = std::getenv("DGFEM_KERNEL");
     const std::string kind = env && env[0] ? env : "valu";
-    if (kind != "valu" && kind != "mm" && kind != "wave")
-        throw UnsupportedError("DGFEM_KERNEL=" + kind + " is not a stage-kernel variant (valu, mm, wave)");
+    if (kind != "valu" && kind != "mm" && kind != "wave" && kind != "bf16x6" && kind != "bf16x3")
+        throw UnsupportedError("DGFEM_KERNEL=" + kind +
+                               " is not a stage-kernel variant (valu, mm, wave, bf16x6, bf16x3)");
+    if (kind == "bf16x6" || kind == "bf16x3") {
+        if (c->precision != DG_MIXED)
+            throw UnsupportedError("DGFEM_KERNEL=" + kind + " is a mixed-precision kernel (FP32 state)");
+        const int nsplit = kind == "bf16x6" ? 3 : 2;
+        if (!dgk::stageBf16Built(DIM, P, NF, nsplit))
+            throw UnsupportedError("DGFEM_KERNEL=" + kind + " is built for tetrahedra of order 2..5 only");
+        if constexpr (DIM == 3 && P >= 2) {
+            if (nsplit == 3)
+                pickBf16<DIM, P, NF, 3>(c);
+            else
+                pickBf16<DIM, P, NF, 2>(c);
+            return true;
+        }
+        throw UnsupportedError("DGFEM_KERNEL=" + kind + " is built for tetrahedra of order 2..5 only");
+    }
     if (kind == "mm" && !(NF == 4 && dgk::BlockMmShape<DIM, P>::kSupported))
         throw UnsupportedError("DGFEM_KERNEL=mm is built for 4 fields on tetrahedra of order <= 3 and triangles "
                                "of order <= 5 only");
@@ -789,7 +890,7 @@ void destroyContext(dg_context* c) {
     if (c->device >= 0) (void)hipSetDevice(c->device);
     releaseGraphs(c);
     freeTables(c);
-    for (void* p : {c->u, c->ulo, c->qa, c->qb, c->acc, c->snap, c->geo, c->fgeo, c->ops, c->afrag})
+    for (void* p : {c->u, c->ulo, c->qa, c->qb, c->acc, c->snap, c->geo, c->fgeo, c->ops, c->afrag, c->afragBf})
         devFree(p);
     devFree(c->nbr);
     devFree(c->iops);
@@ -895,6 +996,12 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         HIPCHK(hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming));
         c->nblocks = (c->K + c->eb - 1) / c->eb;  // one batch of eb elements per block
+        if (c->bfGroups > 0) {
+            // persistent blocks of the BF16 matrix-core kernel: one per CU at most
+            hipDeviceProp_t prop;
+            HIPCHK(hipGetDeviceProperties(&prop, c->device));
+            c->nblocks = std::min<int64_t>((c->bfGroups + c->bfWaves - 1) / c->bfWaves, prop.multiProcessorCount);
+        }
         const size_t nstate = static_cast<size_t>(c->K) * c->Np * c->nf;
         auto alloc = [&](size_t n) -> void* {
             return c->precision == DG_FP64 ? static_cast<void*>(devAlloc<double>(c, n))
@@ -934,6 +1041,7 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         buildOps(R, ops, iops);
         c->ops = alloc(ops.size());
         if (c->mfma) c->afrag = alloc(c->afragHost.size());
+        if (!c->afragBfHost.empty()) c->afragBf = uploadVec(c, c->afragBfHost);
         if (c->precision == DG_FP64) {
             uploadAs<double>(c, c->geo, geo);
             uploadAs<double>(c, c->fgeo, fgeo);

@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import os
 import time
 
 import numpy as np
@@ -111,12 +112,16 @@ def run_closure(dim, steps, warmup=10, seed=5):
     return out
 
 
-def run(name, steps, warmup=10):
+def run(name, steps, warmup=10, mixed_kernels=("valu",), fp64=True):
+    """FP64 (unless fp64 is False) and mixed precision with each stage kernel of mixed_kernels
+    (DGFEM_KERNEL values), from the same initial state; errors against FP64."""
     w = WORKLOADS[name]()
     m = w["mesh"]
     out = dict(workload=name, describe=w["describe"], steps=steps)
     states = {}
-    for label, prec in (("fp64", dg.DG_FP64), ("mixed", dg.DG_MIXED)):
+    runs = ([("fp64", dg.DG_FP64, "valu")] if fp64 else []) + [("mixed_" + k, dg.DG_MIXED, k) for k in mixed_kernels]
+    for label, prec, kernel in runs:
+        os.environ["DGFEM_KERNEL"] = kernel
         s = dg.Solver(m.node_coords, m.element_vertices, w["order"], w["bc"], v0=w["v0"], rho0=RHO0, c0=w["c0"],
                       fields=w["fields"], precision=prec, steps_per_graph=20)
         info = s.info()
@@ -128,18 +133,24 @@ def run(name, steps, warmup=10):
         r = s.run_steps(steps, dt)
         wall = time.perf_counter() - t0
         dofs = info.num_elements * info.nodes_per_element * s.nf
-        stage, _, post = s.profile_steps(dt, 5)
-        states[label] = (s.get_state(), r.observers)
+        u = s.get_state()
+        states[label] = (u[0].copy(), r.observers[:, :, 0].copy() if r.observers.size else None)
+        del u
+        stage, _, post = s.profile_steps(dt, 5)  # advances the state (taken above)
         out[label] = dict(dof_updates_per_s=dofs * steps / (r.device_ms * 1e-3), ms_per_step=r.device_ms / steps,
                           wall_ms_per_step=1e3 * wall / steps, stage_ms=[float(v) for v in stage],
-                          post_ms=float(post), dt=dt, dofs=int(dofs), device_bytes=int(info.device_bytes))
+                          post_ms=float(post), dt=dt, dofs=int(dofs), device_bytes=int(info.device_bytes),
+                          kernel=dg.KERNEL_NAMES[info.stage_kernel])
         s.close()
         print(f"# {label}: {out[label]['dof_updates_per_s']:.4e} DOF-updates/s, {out[label]['ms_per_step']:.4f} "
               f"ms/step", flush=True)
-    up, _ = states["fp64"]
-    um, _ = states["mixed"]
-    out["rel_l2_pressure_mixed_vs_fp64"] = float(np.linalg.norm(um[0] - up[0]) / np.linalg.norm(up[0]))
-    out["speedup_mixed_vs_fp64"] = out["mixed"]["dof_updates_per_s"] / out["fp64"]["dof_updates_per_s"]
+    os.environ.pop("DGFEM_KERNEL", None)
+    if fp64:
+        up, _ = states["fp64"]
+        for label, _, _ in runs[1:]:
+            um, _ = states[label]
+            out[label]["rel_l2_pressure_vs_fp64"] = float(np.linalg.norm(um - up) / np.linalg.norm(up))
+            out[label]["speedup_vs_fp64"] = out[label]["dof_updates_per_s"] / out["fp64"]["dof_updates_per_s"]
     return out
 
 
@@ -147,11 +158,13 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["closure2d", "closure3d"], default="dense3d")
     ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--mixed-kernels", default="valu", help="comma-separated DGFEM_KERNEL values for the mixed runs")
+    ap.add_argument("--no-fp64", action="store_true")
     a = ap.parse_args(argv)
     if a.workload.startswith("closure"):
         res = run_closure(int(a.workload[7]), a.steps)
     else:
-        res = run(a.workload, a.steps)
+        res = run(a.workload, a.steps, mixed_kernels=tuple(a.mixed_kernels.split(",")), fp64=not a.no_fp64)
     print(json.dumps(res), flush=True)
 
 

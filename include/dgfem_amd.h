@@ -43,9 +43,11 @@ enum { DG_FIELDS_REF4 = 0, DG_FIELDS_LEE5 = 1 };    /* (p,vx,vy,vz) or + rho */
 enum { DG_BC_ABSORBING = 0, DG_BC_REFLECTING = 1, DG_BC_LEARNED = 2 };
 enum { DG_RK4 = 0, DG_EULER = 1 };                  /* config timeIntMethod */
 /* Stage-kernel variants (dg_info.stage_kernel): the VALU kernel (default), the block and the
-   wave-autonomous FP32/FP64 matrix-core kernels (environment DGFEM_KERNEL = valu | mm | wave;
-   an explicit request that is not built for the mesh fails dg_create with DG_ERR_UNSUPPORTED). */
-enum { DG_KERNEL_VALU = 0, DG_KERNEL_MM = 1, DG_KERNEL_WAVE = 2 };
+   wave-autonomous FP32/FP64 matrix-core kernels, and the split-BF16 matrix-core kernels of the
+   mixed precision (operands as 3 BF16 terms, six products: FP32 operand precision; or 2 terms,
+   three products). Environment DGFEM_KERNEL = valu | mm | wave | bf16x6 | bf16x3; an explicit
+   request that is not built for the mesh or precision fails dg_create with DG_ERR_UNSUPPORTED. */
+enum { DG_KERNEL_VALU = 0, DG_KERNEL_MM = 1, DG_KERNEL_WAVE = 2, DG_KERNEL_BF16X6 = 3, DG_KERNEL_BF16X3 = 4 };
 
 typedef struct dg_context dg_context;
 

@@ -3,8 +3,8 @@
 Configured size: [-1,1]^3 as 44^3 hexes x 6 jittered Kuhn tets (511,104 tets, 143M DOF), a
 seeded sum of 8 Gaussian pulses, reflecting at z=-1 and absorbing elsewhere (physical groups in
 ascending tag order: absorbing, then reflecting), CFL-limited dt, 2000 RK4 steps: mixed
-precision within 1e-6 of FP64 in pressure (field and observer series), and the density stays
-p/c0^2 (the isentropic invariant of the 5-field variant).
+precision within 1e-6 of FP64 in pressure (field and observer series) and in the density, and
+the FP64 density stays p/c0^2 (the isentropic invariant of the 5-field variant) to rounding.
 
 At reduced size the 5-field variant's (p, v) must equal the 4-field oracle (the reference
 algorithm restated) to 1e-10: the density is passive and does not feed back.
@@ -91,15 +91,18 @@ def test_dense_case_2000_steps_mixed_within_1e6_of_fp64():
         assert r.steps == 2000
         u = s.get_state()
         out[prec] = (u[0].copy(), r.observers[:, :, 0].copy(), rel_l2(u[4], u[0] / C0 ** 2),
-                     [rel_l2(u[f], w["u0"][f]) for f in (0,)])
+                     rel_l2(u[0], w["u0"][0]), u[4].copy())
         print(f"precision {prec}: {r.device_ms / 2000:.3f} ms/step, rho invariant {out[prec][2]:.2e}", flush=True)
         del u
         s.close()
-    p64, o64, inv64, _ = out[dg.DG_FP64]
-    pmx, omx, invmx, _ = out[dg.DG_MIXED]
+    p64, o64, inv64, moved, r64 = out[dg.DG_FP64]
+    pmx, omx, invmx, _, rmx = out[dg.DG_MIXED]
     e = rel_l2(pmx, p64)
     eo = rel_l2(omx, o64)
-    print(f"dense case mixed vs fp64 after 2000 steps: pressure {e:.2e}, observers {eo:.2e}")
-    assert inv64 < 1e-12 and invmx < 1e-6
-    assert out[dg.DG_FP64][3][0] > 1e-2  # the field has evolved
-    assert e <= 1e-6 and eo <= 1e-6
+    er = rel_l2(rmx, r64)
+    print(f"dense case mixed vs fp64 after 2000 steps: pressure {e:.2e}, observers {eo:.2e}, density {er:.2e}")
+    # FP64 keeps rho = p / c0^2 to rounding; in mixed precision rho and p each carry their own
+    # rounding, so rho is held to the same bar against the FP64 rho as the pressure
+    assert inv64 < 1e-12
+    assert moved > 1e-2  # the field has evolved
+    assert e <= 1e-6 and eo <= 1e-6 and er <= 1e-6
