@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -24,6 +26,38 @@
 namespace {
 
 thread_local std::string g_createError;
+
+// Staging memory of dg_face_traces_noalloc: one array per device, held until the process ends,
+// shared by every call on that device (overlapping calls on one device race). Host code uses
+// only the per-device address obtained with hipGetSymbolAddress, never the array's name.
+constexpr size_t kScratchBytes = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+std::array<void*, kMaxDevices> g_scratchPtr{};
+
+// Device address of the staging array on `device` (which must be current), looked up once per
+// device.
+hipError_t scratchAddress(int device, void** out) {
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    if (!g_scratchPtr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratchPtr[device] = ptr;
+    }
+    *out = g_scratchPtr[device];
+    return hipSuccess;
+}
+
+size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+// Staging bytes of dg_face_traces_noalloc for n faces: element ids, local faces, traces, normals.
+size_t traceScratchBytes(int64_t n, int nfp, int nf) {
+    const size_t m = static_cast<size_t>(n);
+    return align256(8 * m) + align256(4 * m) + align256(16 * m * nfp * nf) + align256(24 * m);
+}
 
 
 // A stage-kernel variant that was asked for but is not built for the mesh.
@@ -120,6 +154,7 @@ struct dg_context {
     // per-run settings read by the launchers
     int curResid = 1;
     double curDt = 0.0, gDt = -1.0;
+    void* scratch = nullptr;  // this device's staging array (dg_face_traces_noalloc)
 };
 
 namespace {
@@ -991,6 +1026,7 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         }
         c->device = c->opt.device;
         HIPCHK(hipSetDevice(c->device));
+        HIPCHK(scratchAddress(c->device, &c->scratch));
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming));
@@ -1148,6 +1184,40 @@ dg_status dg_face_traces(dg_context* c, int64_t n, const int64_t* elem, const in
     }
     for (void* p : {dE, dF, dO, dN}) devFree(p);
     return st;
+}
+
+dg_status dg_face_traces_noalloc(dg_context* c, int64_t n, const int64_t* elem, const int32_t* face, double* out,
+                                 double* normals) {
+    if (!c || n < 0 || (n > 0 && (!elem || !face || !out || !normals))) return DG_ERR_INVALID;
+    if (traceScratchBytes(n, c->Nfp, c->nf) > kScratchBytes)
+        return fail(c, DG_ERR_INVALID,
+                    "dg_face_traces_noalloc: " + std::to_string(n) + " faces need " +
+                        std::to_string(traceScratchBytes(n, c->Nfp, c->nf)) + " bytes of staging, more than the " +
+                        std::to_string(kScratchBytes) + " of the per-device array");
+    for (int64_t i = 0; i < n; ++i)
+        if (elem[i] < 0 || elem[i] >= c->K || face[i] < 0 || face[i] >= c->Nfc)
+            return fail(c, DG_ERR_INVALID, "dg_face_traces_noalloc: element or face out of range");
+    if (n == 0) return DG_OK;
+    const size_t m = static_cast<size_t>(n);
+    unsigned char* base = static_cast<unsigned char*>(c->scratch);
+    int64_t* dE = reinterpret_cast<int64_t*>(base);
+    int32_t* dF = reinterpret_cast<int32_t*>(base + align256(8 * m));
+    double* dO = reinterpret_cast<double*>(base + align256(8 * m) + align256(4 * m));
+    double* dN = reinterpret_cast<double*>(base + align256(8 * m) + align256(4 * m) + align256(16 * m * c->Nfp * c->nf));
+    const size_t outN = m * 2 * c->Nfp * c->nf;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipMemcpyAsync(dE, elem, m * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(dF, face, m * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        c->traceFn(c, n, dE, dF, dO, dN);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, dO, outN * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(normals, dN, m * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const std::exception& ex) {
+        return fail(c, DG_ERR_DEVICE, ex.what());
+    }
+    return DG_OK;
 }
 
 const char* dg_last_error(const dg_context* c) { return c ? c->err.c_str() : g_createError.c_str(); }

@@ -124,6 +124,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "dg_learned_faces": (C.c_int32, [C.c_void_p, P(C.c_int64), P(C.c_int32), C.c_int64, P(C.c_int64)]),
         "dg_face_traces": (C.c_int32, [C.c_void_p, C.c_int64, P(C.c_int64), P(C.c_int32), P(C.c_double),
                                        P(C.c_double)]),
+        "dg_face_traces_noalloc": (C.c_int32, [C.c_void_p, C.c_int64, P(C.c_int64), P(C.c_int32), P(C.c_double),
+                                               P(C.c_double)]),
         "dg_closure_num_params": (C.c_int64, [C.c_int32, C.c_int32]),
         "dg_closure_last_error": (C.c_char_p, []),
         "dg_closure_to_bf16": (None, [P(C.c_float), C.c_int64, P(C.c_uint16)]),

@@ -196,6 +196,17 @@ DG_API dg_status dg_learned_faces(const dg_context* ctx, int64_t* elem, int32_t*
    boundary); normals [n][3] the outward unit normals. */
 DG_API dg_status dg_face_traces(dg_context* ctx, int64_t n, const int64_t* elem, const int32_t* face, double* out,
                                 double* normals);
+/* No counterpart in the reference: dg_face_traces for callers that cannot allocate device
+   memory (code under graph capture, tight per-step loops such as the closure's training-data
+   collection, harnesses that forbid hipMalloc). Same arguments, results and checks; the device
+   staging (element ids, faces, traces, normals) lives in a fixed 8 MiB array per device, held
+   until the process ends, and the call allocates nothing. It accepts n faces when
+   align256(8n) + align256(4n) + align256(16 n Nfp fields) + align256(24n) <= 8 MiB
+   (align256: rounded up to a multiple of 256; 12,408 faces of order-3 tetrahedra with 4
+   fields), else returns DG_ERR_INVALID. Calls on one device share the array: calls that
+   overlap in time on one device (several host threads) race. */
+DG_API dg_status dg_face_traces_noalloc(dg_context* ctx, int64_t n, const int64_t* elem, const int32_t* face,
+                                        double* out, double* normals);
 
 DG_API int64_t dg_closure_num_params(int32_t n_in, int32_t n_out);
 DG_API const char* dg_closure_last_error(void);
