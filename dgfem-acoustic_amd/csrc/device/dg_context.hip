@@ -60,6 +60,9 @@ size_t traceScratchBytes(int64_t n, int nfp, int nf) {
 }
 
 
+// Lowest tetrahedral order whose mixed-precision default is the split-BF16 kernel.
+constexpr int kBf16DefaultMinOrder = 5;
+
 // A stage-kernel variant that was asked for but is not built for the mesh.
 struct UnsupportedError : std::runtime_error {
     using std::runtime_error::runtime_error;
@@ -137,7 +140,7 @@ struct dg_context {
     void* afragBf = nullptr;  // split-BF16 operator fragments (uint16) of the BF16 matrix-core kernel
     std::vector<uint16_t> afragBfHost;
     int64_t bfGroups = 0;     // 16-element groups of the BF16 matrix-core kernel (persistent grid), or 0
-    int bfWaves = 0;          // its waves per block
+    int bfBlocksPerCu = 0;    // its resident blocks per CU
     int bfSplit = 0;          // BF16 terms per operand (2 or 3)
     std::vector<double> afragHost;
     PerStepFn sourceFn = nullptr, postFn = nullptr;
@@ -346,7 +349,7 @@ double fromBf16(uint16_t b) {
 // into NSPLIT BF16 terms, each the nearest BF16 of what the previous terms leave.
 template <int DIM, int P, int NSPLIT>
 void buildAfragBf16(dg_context* c, const dg::RefElement& R) {
-    using B = dgk::Bf16Shape<DIM, P>;
+    using B = dgk::Bf16Shape<DIM, P, 4>;  // the fragment layout does not depend on the fields
     const int Np = R.Np, NFT = R.Nfaces * R.Nfp;
     c->afragBfHost.assign(static_cast<size_t>(B::MT) * B::KS * NSPLIT * 64 * 8, 0);
     for (int it = 0; it < B::MT; ++it)
@@ -510,14 +513,25 @@ void pickAux(dg_context* c) {
     }
 }
 
+// Stage kernel without DGFEM_KERNEL: the split-BF16 matrix-core kernel for the mixed precision
+// where it measured faster than the VALU kernel (tetrahedra of order 5: 5.07 against 7.77 ms per
+// step on the 511,104-tet 5-field case, at lower error against FP64); the VALU kernel elsewhere
+// (order 3: 0.62 against 0.40 ms per step on the 196,608-tet 4-field case, whose contractions are
+// too small a part of a stage for the split operands to pay).
+std::string defaultKernel(int dim, int p, int nf, int precision) {
+    if (precision == DG_MIXED && dim == 3 && p >= kBf16DefaultMinOrder && dgk::stageBf16Built(dim, p, nf, 3))
+        return "bf16x6";
+    return "valu";
+}
+
 template <int DIM, int P, int NF, int NSPLIT>
 void pickBf16(dg_context* c) {
     c->mfma = false;
     c->kernelKind = NSPLIT == 3 ? DG_KERNEL_BF16X6 : DG_KERNEL_BF16X3;
     buildAfragBf16<DIM, P, NSPLIT>(c, dg::refElement(DIM, P));
-    c->eb = dgk::Bf16Shape<DIM, P>::kGroup;
+    c->eb = dgk::Bf16Shape<DIM, P, NF>::kGroup;
     c->bfGroups = (c->K + c->eb - 1) / c->eb;
-    c->bfWaves = dgk::Bf16Shape<DIM, P>::kWaves;
+    c->bfBlocksPerCu = dgk::stageBf16BlocksPerCu(DIM, P, NF, NSPLIT);
     c->bfSplit = NSPLIT;
     c->stageFn = &launchStageBf16;
     c->sourceFn = &launchSource<float, true>;
@@ -533,7 +547,7 @@ template <int DIM, int P, int NF>
 bool pick(dg_context* c) {
     pickAux<DIM, P, NF>(c);
     const char* env = std::getenv("DGFEM_KERNEL");
-    const std::string kind = env && env[0] ? env : "valu";
+    const std::string kind = env && env[0] ? env : defaultKernel(DIM, P, NF, c->precision);
     if (kind != "valu" && kind != "mm" && kind != "wave" && kind != "bf16x6" && kind != "bf16x3")
         throw UnsupportedError("DGFEM_KERNEL=" + kind +
                                " is not a stage-kernel variant (valu, mm, wave, bf16x6, bf16x3)");
@@ -1033,10 +1047,10 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_physics* ph, const dg_option
         HIPCHK(hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming));
         c->nblocks = (c->K + c->eb - 1) / c->eb;  // one batch of eb elements per block
         if (c->bfGroups > 0) {
-            // persistent blocks of the BF16 matrix-core kernel: one per CU at most
+            // persistent blocks of the BF16 matrix-core kernel: as many as are resident at once
             hipDeviceProp_t prop;
             HIPCHK(hipGetDeviceProperties(&prop, c->device));
-            c->nblocks = std::min<int64_t>((c->bfGroups + c->bfWaves - 1) / c->bfWaves, prop.multiProcessorCount);
+            c->nblocks = std::min<int64_t>(c->bfGroups, static_cast<int64_t>(prop.multiProcessorCount) * c->bfBlocksPerCu);
         }
         const size_t nstate = static_cast<size_t>(c->K) * c->Np * c->nf;
         auto alloc = [&](size_t n) -> void* {

@@ -16,6 +16,8 @@ precision from the same initial state; mixed-precision error reported against FP
            with a nonzero output layer (timing only, not trained).
   closure2d: the same at the 2D configuration's truncated size: [-1,1]^2 as 100^2 x 2 jittered
            triangles (20,000 triangles, 400 boundary faces), p=4.
+  box3d:   [-1,1]^3 as n^3 x 6 jittered Kuhn tets at a given order (--n, --order), 4 fields,
+           absorbing, a Gaussian pulse: stage-kernel comparisons across orders.
 
     python dgfem-acoustic_amd/workloads.py --workload dense3d --steps 200
 """
@@ -65,7 +67,19 @@ def lee2d(n=96, seed=1):
                          "CFL-limited dt")
 
 
-WORKLOADS = {"dense3d": dense3d, "lee2d": lee2d}
+def box3d(n=32, order=3, seed=2):
+    """[-1,1]^3 as n^3 x 6 jittered Kuhn tets at the given order, 4 fields, v0=(30,0,0), c0=340,
+    absorbing, a Gaussian pulse exp(-r^2/0.01) at the origin (kernel comparisons across orders)."""
+    m = dg.box_mesh(3, n, order, jitter=0.15, seed=seed)
+    bc = dg.resolve_bc_sides(m, [(0x3F, dg.DG_BC_ABSORBING)])
+    X = m.node_coords.reshape(-1, 3)
+    u = np.zeros((4, len(X)))
+    u[0] = np.exp(-(X ** 2).sum(axis=1) / 0.01)
+    return dict(mesh=m, bc=bc, order=order, v0=(30.0, 0.0, 0.0), c0=340.0, fields=dg.DG_FIELDS_REF4, u0=u,
+                describe=f"3D APE, {n}^3x6 jittered Kuhn tets (K={m.K}), p={order}, absorbing, Gaussian pulse")
+
+
+WORKLOADS = {"dense3d": dense3d, "lee2d": lee2d, "box3d": box3d}
 
 
 def run_closure(dim, steps, warmup=10, seed=5):
@@ -112,10 +126,10 @@ def run_closure(dim, steps, warmup=10, seed=5):
     return out
 
 
-def run(name, steps, warmup=10, mixed_kernels=("valu",), fp64=True):
+def run(name, steps, warmup=10, mixed_kernels=("valu",), fp64=True, **kw):
     """FP64 (unless fp64 is False) and mixed precision with each stage kernel of mixed_kernels
     (DGFEM_KERNEL values), from the same initial state; errors against FP64."""
-    w = WORKLOADS[name]()
+    w = WORKLOADS[name](**kw)
     m = w["mesh"]
     out = dict(workload=name, describe=w["describe"], steps=steps)
     states = {}
@@ -160,11 +174,14 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--mixed-kernels", default="valu", help="comma-separated DGFEM_KERNEL values for the mixed runs")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--n", type=int, default=None, help="box3d: hexes per side")
+    ap.add_argument("--order", type=int, default=None, help="box3d: Lagrange order")
     a = ap.parse_args(argv)
     if a.workload.startswith("closure"):
         res = run_closure(int(a.workload[7]), a.steps)
     else:
-        res = run(a.workload, a.steps, mixed_kernels=tuple(a.mixed_kernels.split(",")), fp64=not a.no_fp64)
+        kw = {k: v for k, v in (("n", a.n), ("order", a.order)) if v is not None}
+        res = run(a.workload, a.steps, mixed_kernels=tuple(a.mixed_kernels.split(",")), fp64=not a.no_fp64, **kw)
     print(json.dumps(res), flush=True)
 
 

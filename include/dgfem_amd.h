@@ -46,7 +46,8 @@ enum { DG_RK4 = 0, DG_EULER = 1 };                  /* config timeIntMethod */
    wave-autonomous FP32/FP64 matrix-core kernels, and the split-BF16 matrix-core kernels of the
    mixed precision (operands as 3 BF16 terms, six products: FP32 operand precision; or 2 terms,
    three products). Environment DGFEM_KERNEL = valu | mm | wave | bf16x6 | bf16x3; an explicit
-   request that is not built for the mesh or precision fails dg_create with DG_ERR_UNSUPPORTED. */
+   request that is not built for the mesh or precision fails dg_create with DG_ERR_UNSUPPORTED.
+   Without it: bf16x6 for the mixed precision on tetrahedra of order 5, valu otherwise. */
 enum { DG_KERNEL_VALU = 0, DG_KERNEL_MM = 1, DG_KERNEL_WAVE = 2, DG_KERNEL_BF16X6 = 3, DG_KERNEL_BF16X3 = 4 };
 
 typedef struct dg_context dg_context;

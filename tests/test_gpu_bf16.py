@@ -110,3 +110,36 @@ def test_bf16_unsupported_requests(monkeypatch):
         dg.Solver(m2.node_coords, m2.element_vertices, 3, bc2, v0=(0, 0, 0), rho0=RHO0, c0=340.0,
                   precision=dg.DG_MIXED)
     assert ei.value.code == dg.DG_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("p,fields", [(3, dg.DG_FIELDS_REF4), (5, dg.DG_FIELDS_LEE5)])
+def test_bf16_blocks_with_several_groups(p, fields, monkeypatch):
+    """12^3 x 6 tets (648 groups of 16 elements, more than the persistent blocks, so blocks run
+    several groups and prefetch the next one), reflecting floor + absorbing, 60 RK4 steps with a
+    monopole and an observer: bf16x6 within 1e-6 of FP64 in pressure (field and observer)."""
+    print(f"case bf16 several groups per block p={p}", flush=True)
+    groups = [(0x2F, 0), (0x10, 1)]
+    v0 = (30.0, -10.0, 5.0)
+    m, bc, gn = make_case(3, 12, p, groups, jitter=0.15, seed=19)
+    nf = 5 if fields == dg.DG_FIELDS_LEE5 else 4
+    u0 = gaussian_state(m, 0.1, -0.1, 0.2, 0.02, nf=nf, c0=340.0)
+    res = {}
+    for prec, kernel in ((dg.DG_FP64, "valu"), (dg.DG_MIXED, "bf16x6")):
+        monkeypatch.setenv("DGFEM_KERNEL", kernel)
+        s = dg.Solver(m.node_coords, m.element_vertices, p, bc, v0=v0, rho0=RHO0, c0=340.0, fields=fields,
+                      precision=prec, steps_per_graph=10)
+        info = s.info()
+        if kernel == "bf16x6":
+            assert info.stage_blocks < (m.K + 15) // 16
+        dt = 0.5 * info.min_inradius / ((340.0 + 35.0) * (2 * p + 1))
+        s.set_state(u0)
+        s.add_monopole(0.2, 0.0, 0.0, 0.1, 0.1, 300.0, 0.0, 1.0)
+        s.add_observer(-0.4, 0.2, 0.0, 0.1)
+        r = s.run_steps(60, dt)
+        res[prec] = (s.get_state(), r.observers, r.residuals)
+        s.close()
+    (u64, o64, r64), (u, o, r) = res[dg.DG_FP64], res[dg.DG_MIXED]
+    e, eo = rel_l2(u[0], u64[0]), rel_l2(o[:, :, 0], o64[:, :, 0])
+    print(f"pressure {e:.2e}, observer {eo:.2e}")
+    assert e < 1e-6 and eo < 1e-6
+    assert np.abs(r[:, :4] / r64[:, :4] - 1.0).max() < 1e-4
