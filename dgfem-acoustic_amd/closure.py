@@ -270,6 +270,79 @@ def scale_output(master: np.ndarray, nfp: int, alpha: float, dim: int = 2) -> np
     return m
 
 
+def collect_training(case: ClosureCase, meshes: Meshes, dt: float, steps: int, n_train: int, record_every: int = 2,
+                     face_fraction: float = 1.0, log=print):
+    """FP64 enlarged-domain training runs (seeds 1000+i) -> samples (X, Y)."""
+    Xs, Ys = [], []
+    for i in range(n_train):
+        scen = scenario(case, 1000 + i)
+        t0 = time.perf_counter()
+        _, X, Y = enlarged_run(case, meshes, scen, dt, steps, record_every, face_fraction, seed=i)
+        Xs.append(X)
+        Ys.append(Y)
+        log(f"  training run {i + 1}/{n_train} ({scen['kind']}): {X.shape[0]} samples "
+            f"({time.perf_counter() - t0:.1f} s)")
+    X = np.concatenate(Xs)
+    Y = np.concatenate(Ys)
+    return X, Y
+
+
+def train_network(case: ClosureCase, X, Y, epochs: int, batch: int, lr: float, log=print):
+    """Adam on the GPU from the seeded initial network; returns (master, loss history, seconds)."""
+    master = dg.closure_init(face_nodes(case), seed=case.seed, dim=case.dim)
+    t0 = time.perf_counter()
+    master, _, hist = dg.closure_train(master, X, Y, epochs=epochs, batch=batch, lr=lr, seed=case.seed)
+    t_train = time.perf_counter() - t0
+    log(f"  trained {epochs} epochs on {X.shape[0]} samples in {t_train:.1f} s: loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    return master, [float(v) for v in hist], t_train
+
+
+def face_nodes(case: ClosureCase) -> int:
+    return case.order + 1 if case.dim == 2 else (case.order + 1) * (case.order + 2) // 2
+
+
+def choose_alpha(case: ClosureCase, meshes: Meshes, dt: float, steps: int, master, n_val: int,
+                 alphas=(0.0, 0.25, 0.5, 0.75, 1.0), log=print):
+    """Output blend chosen on validation runs (seeds 3000+j) by the mean observer error."""
+    val = []
+    for j in range(n_val):
+        scen = scenario(case, 3000 + j)
+        ref, _, _ = enlarged_run(case, meshes, scen, dt, steps, record_every=0)
+        val.append((scen, ref))
+    best, table = None, {}
+    for a in alphas:
+        w = dg.to_bf16(scale_output(master, face_nodes(case), a, case.dim))
+        errs = [rel_l2(truncated_run(case, meshes, sc, dt, steps, "learned", w)[0], ref) for sc, ref in val]
+        table[str(a)] = errs
+        log(f"  validation alpha={a}: observer rel L2 {['%.3e' % e for e in errs]}")
+        if best is None or np.mean(errs) < best[0]:
+            best = (float(np.mean(errs)), a)
+    return best[1], table
+
+
+def evaluate(case: ClosureCase, meshes: Meshes, dt: float, steps: int, params, seeds, log=print) -> list:
+    """Held-out runs: the reference's characteristic BC, the exact characteristic condition
+    (zero network) and the learned closure on the truncated domain, each against the
+    enlarged-domain FP64 solve (observer pressure rel-L2). One dict per seed."""
+    out_all = []
+    for j, seed in enumerate(seeds):
+        scen = scenario(case, seed)
+        t0 = time.perf_counter()
+        ref, _, _ = enlarged_run(case, meshes, scen, dt, steps, record_every=0)
+        t_ref = time.perf_counter() - t0
+        out = dict(seed=int(seed), kind=scen["kind"], v0=[float(v) for v in scen["v0"]], enlarged_seconds=t_ref)
+        for name, mode, prm in (("characteristic", "characteristic", None),
+                                ("exact_characteristic", "learned", None), ("learned", "learned", params)):
+            o, ms = truncated_run(case, meshes, scen, dt, steps, mode, prm)
+            out[name] = rel_l2(o, ref)
+            out[name + "_ms_per_step"] = ms / steps
+        out_all.append(out)
+        log(f"  held-out {j + 1}/{len(seeds)} (seed {seed}, {scen['kind']}): observer rel L2 vs enlarged: "
+            f"characteristic {out['characteristic']:.3e} exact-characteristic {out['exact_characteristic']:.3e} "
+            f"learned {out['learned']:.3e}")
+    return out_all
+
+
 def train_and_evaluate(case: ClosureCase, n_train: int, n_eval: int, n_val: int = 2, epochs: int = 60,
                        batch: int = 2048, lr: float = 1e-3, record_every: int = 2, face_fraction: float = 1.0,
                        alphas=(0.0, 0.25, 0.5, 0.75, 1.0), log=print) -> Report:
@@ -281,61 +354,38 @@ def train_and_evaluate(case: ClosureCase, n_train: int, n_eval: int, n_val: int 
     dt, steps = time_step(case, meshes)
     log(f"closure case: enlarged K={meshes.parent_verts.shape[0]} truncated K={meshes.sub_verts.shape[0]} "
         f"truncation faces={meshes.faces_sub[0].size} dt={dt:.3e} steps={steps}")
-    Xs, Ys = [], []
-    for i in range(n_train):
-        scen = scenario(case, 1000 + i)
-        _, X, Y = enlarged_run(case, meshes, scen, dt, steps, record_every, face_fraction, seed=i)
-        Xs.append(X)
-        Ys.append(Y)
-        log(f"  training run {i + 1}/{n_train} ({scen['kind']}): {X.shape[0]} samples")
-    X = np.concatenate(Xs)
-    Y = np.concatenate(Ys)
-    del Xs, Ys
-    nfp = case.order + 1 if case.dim == 2 else (case.order + 1) * (case.order + 2) // 2
-    master = dg.closure_init(nfp, seed=case.seed, dim=case.dim)
-    t0 = time.perf_counter()
-    master, _, hist = dg.closure_train(master, X, Y, epochs=epochs, batch=batch, lr=lr, seed=case.seed)
-    t_train = time.perf_counter() - t0
-    log(f"  trained {epochs} epochs on {X.shape[0]} samples in {t_train:.1f} s: loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    X, Y = collect_training(case, meshes, dt, steps, n_train, record_every, face_fraction, log)
+    master, hist, t_train = train_network(case, X, Y, epochs, batch, lr, log)
     rep = Report(case=case.__dict__.copy(), dt=dt, steps=steps, train_samples=int(X.shape[0]),
-                 epoch_loss=[float(v) for v in hist], train_seconds=t_train)
+                 epoch_loss=hist, train_seconds=t_train)
     del X, Y
-    # output blend chosen on validation runs
-    val = []
-    for j in range(n_val):
-        scen = scenario(case, 3000 + j)
-        ref, _, _ = enlarged_run(case, meshes, scen, dt, steps, record_every=0)
-        val.append((scen, ref))
-    best = None
-    for a in alphas:
-        w = dg.to_bf16(scale_output(master, nfp, a, case.dim))
-        errs = [rel_l2(truncated_run(case, meshes, sc, dt, steps, "learned", w)[0], ref) for sc, ref in val]
-        rep.validation[str(a)] = errs
-        log(f"  validation alpha={a}: observer rel L2 {['%.3e' % e for e in errs]}")
-        if best is None or np.mean(errs) < best[0]:
-            best = (float(np.mean(errs)), a)
-    rep.alpha = best[1]
-    params = dg.to_bf16(scale_output(master, nfp, rep.alpha, case.dim))
+    rep.alpha, rep.validation = choose_alpha(case, meshes, dt, steps, master, n_val, alphas, log)
+    params = dg.to_bf16(scale_output(master, face_nodes(case), rep.alpha, case.dim))
     rep.params = params
     rep.master = master
-    for j in range(n_eval):
-        scen = scenario(case, 5000 + j)
-        ref, _, _ = enlarged_run(case, meshes, scen, dt, steps, record_every=0)
-        out = dict(kind=scen["kind"], v0=[float(v) for v in scen["v0"]])
-        # characteristic: the reference's absorbing BC; exact_characteristic: the zero network
-        for name, mode, prm in (("characteristic", "characteristic", None),
-                                ("exact_characteristic", "learned", None), ("learned", "learned", params)):
-            o, ms = truncated_run(case, meshes, scen, dt, steps, mode, prm)
-            out[name] = rel_l2(o, ref)
-            out[name + "_ms_per_step"] = ms / steps
-        rep.eval.append(out)
-        log(f"  held-out {j + 1}/{n_eval} ({scen['kind']}): observer rel L2 vs enlarged: characteristic "
-            f"{out['characteristic']:.3e} exact-characteristic {out['exact_characteristic']:.3e} "
-            f"learned {out['learned']:.3e}")
+    rep.eval = evaluate(case, meshes, dt, steps, params, [5000 + j for j in range(n_eval)], log)
     return rep
 
 
+def save_network(path: str, master, case: ClosureCase, alpha=None, **info):
+    """FP32 master weights with the case they were trained on (a plain .npz, no pickled objects)."""
+    import json
+    np.savez(path, master=np.asarray(master, dtype=np.float32),
+             alpha=np.float64(np.nan if alpha is None else alpha),
+             meta=np.array(json.dumps(dict(case=case.__dict__, **info))))
+
+
+def load_network(path: str):
+    import json
+    with np.load(path, allow_pickle=False) as z:
+        alpha = float(z["alpha"])
+        return np.array(z["master"], dtype=np.float32), (None if np.isnan(alpha) else alpha), json.loads(str(z["meta"]))
+
+
 def main(argv=None):
+    """Whole pipeline in one process, or split across processes: --train N --save-network F
+    (training only with --eval 0 --val 0), then --load-network F [--alpha A] --eval-seeds ...
+    (validation and held-out runs)."""
     import argparse
     import json
     ap = argparse.ArgumentParser(description="Train the learned absorbing closure on enlarged-domain runs and "
@@ -344,29 +394,71 @@ def main(argv=None):
     ap.add_argument("--n-trunc", type=int, default=100, help="cells per side of [-1,1]^dim (h = 2/n)")
     ap.add_argument("--order", type=int, default=4)
     ap.add_argument("--factor", type=int, default=3, help="enlarged domain [-factor, factor]^dim")
+    ap.add_argument("--jitter", type=float, default=0.2, help="interior vertex jitter, fraction of h")
+    ap.add_argument("--steps", type=int, default=0, help="RK4 steps per run (0: 3 m of travel at c0)")
     ap.add_argument("--source-size", type=float, default=0.05,
                     help="monopole radius (2D) / dipole-quadrupole pole offset (3D)")
     ap.add_argument("--mixed", action="store_true", help="truncated-domain runs in mixed precision")
     ap.add_argument("--train", type=int, default=32)
     ap.add_argument("--val", type=int, default=2)
-    ap.add_argument("--eval", type=int, default=4)
+    ap.add_argument("--eval", type=int, default=4, help="held-out runs, seeds 5000, 5001, ...")
+    ap.add_argument("--eval-seeds", default="", help="comma-separated held-out seeds (overrides --eval)")
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--record-every", type=int, default=2)
     ap.add_argument("--face-fraction", type=float, default=1.0)
     ap.add_argument("--observer-size", type=float, default=0.03)
+    ap.add_argument("--alpha", type=float, default=None, help="output blend (skips the validation runs)")
+    ap.add_argument("--save-network", default="", help="write the trained FP32 master weights (.npz)")
+    ap.add_argument("--load-network", default="", help="skip training; use these weights (.npz)")
     ap.add_argument("--out", default="", help="write the report as JSON here")
     a = ap.parse_args(argv)
+    log = lambda s: print(s, flush=True)
     case = ClosureCase(n_trunc=a.n_trunc, order=a.order, factor=a.factor, observer_size=a.observer_size, dim=a.dim,
-                       source_size=a.source_size, precision=dg.DG_MIXED if a.mixed else dg.DG_FP64)
-    rep = train_and_evaluate(case, a.train, a.eval, a.val, a.epochs, a.batch, a.lr, a.record_every, a.face_fraction,
-                             log=lambda s: print(s, flush=True))
-    out = dict(rep.__dict__)
-    out.pop("params", None)
-    out.pop("master", None)
-    for k in ("characteristic", "exact_characteristic", "learned"):
-        out["mean_" + k] = float(np.mean([e[k] for e in rep.eval]))
+                       source_size=a.source_size, jitter=a.jitter, steps=a.steps,
+                       precision=dg.DG_MIXED if a.mixed else dg.DG_FP64)
+    seeds = [int(v) for v in a.eval_seeds.split(",") if v] or [5000 + j for j in range(a.eval)]
+    split = bool(a.save_network or a.load_network or a.alpha is not None or a.eval_seeds)
+    if not split:
+        rep = train_and_evaluate(case, a.train, a.eval, a.val, a.epochs, a.batch, a.lr, a.record_every,
+                                 a.face_fraction, log=log)
+        out = dict(rep.__dict__)
+        out.pop("params", None)
+        out.pop("master", None)
+    else:
+        t0 = time.perf_counter()
+        meshes = build_meshes(case)
+        dt, steps = time_step(case, meshes)
+        log(f"closure case: enlarged K={meshes.parent_verts.shape[0]} truncated K={meshes.sub_verts.shape[0]} "
+            f"truncation faces={meshes.faces_sub[0].size} dt={dt:.3e} steps={steps} "
+            f"(meshes built in {time.perf_counter() - t0:.1f} s)")
+        out = dict(case=case.__dict__.copy(), dt=dt, steps=steps)
+        alpha = a.alpha
+        if a.load_network:
+            master, saved_alpha, meta = load_network(a.load_network)
+            alpha = saved_alpha if alpha is None else alpha
+            out["network"] = meta
+        else:
+            X, Y = collect_training(case, meshes, dt, steps, a.train, a.record_every, a.face_fraction, log)
+            master, hist, t_train = train_network(case, X, Y, a.epochs, a.batch, a.lr, log)
+            out.update(train_samples=int(X.shape[0]), epoch_loss=hist, train_seconds=t_train)
+            del X, Y
+        if alpha is None and a.val > 0:
+            alpha, out["validation"] = choose_alpha(case, meshes, dt, steps, master, a.val, log=log)
+        if a.save_network:
+            save_network(a.save_network, master, case, alpha, train_runs=a.train, epochs=a.epochs,
+                         record_every=a.record_every, face_fraction=a.face_fraction)
+            log(f"  network written to {a.save_network} (alpha {alpha})")
+        out["alpha"] = alpha
+        if seeds and a.eval > 0 or a.eval_seeds:
+            if alpha is None:
+                raise SystemExit("held-out runs need an output blend: --alpha, --val > 0 or a saved one")
+            params = dg.to_bf16(scale_output(master, face_nodes(case), alpha, case.dim))
+            out["eval"] = evaluate(case, meshes, dt, steps, params, seeds, log)
+    if out.get("eval"):
+        for k in ("characteristic", "exact_characteristic", "learned"):
+            out["mean_" + k] = float(np.mean([e[k] for e in out["eval"]]))
     print(json.dumps({k: out[k] for k in out if k.startswith("mean_") or k in ("alpha", "steps", "dt",
                                                                               "train_samples")}), flush=True)
     if a.out:

@@ -52,6 +52,7 @@ def dense3d(n=44, seed=3):
     u[0] = p
     u[4] = p / (c0 * c0)
     return dict(mesh=m, bc=bc, order=5, v0=(30.0, 0.0, 0.0), c0=c0, fields=dg.DG_FIELDS_LEE5, u0=u,
+                observers=[(0.5, 0.0, 0.0, 0.05), (0.0, -0.5, 0.25, 0.05), (-0.3, 0.3, -0.9, 0.05)],
                 describe=f"3D LEE, {n}^3x6 jittered Kuhn tets (K={m.K}), p=5, 5 fields, 8 Gaussian pulses, "
                          "reflecting z=-1 + absorbing, CFL-limited dt")
 
@@ -138,6 +139,8 @@ def run(name, steps, warmup=10, mixed_kernels=("valu",), fp64=True, **kw):
         os.environ["DGFEM_KERNEL"] = kernel
         s = dg.Solver(m.node_coords, m.element_vertices, w["order"], w["bc"], v0=w["v0"], rho0=RHO0, c0=w["c0"],
                       fields=w["fields"], precision=prec, steps_per_graph=20)
+        for ob in w.get("observers", ()):
+            s.add_observer(*ob)
         info = s.info()
         dt = info.min_inradius / ((w["c0"] + np.linalg.norm(w["v0"])) * (2 * w["order"] + 1))
         s.set_state(w["u0"])
@@ -164,6 +167,9 @@ def run(name, steps, warmup=10, mixed_kernels=("valu",), fp64=True, **kw):
         for label, _, _ in runs[1:]:
             um, _ = states[label]
             out[label]["rel_l2_pressure_vs_fp64"] = float(np.linalg.norm(um - up) / np.linalg.norm(up))
+            om, op = states[label][1], states["fp64"][1]
+            if op is not None:
+                out[label]["rel_l2_observer_vs_fp64"] = float(np.linalg.norm(om - op) / np.linalg.norm(op))
             out[label]["speedup_vs_fp64"] = out[label]["dof_updates_per_s"] / out["fp64"]["dof_updates_per_s"]
     return out
 

@@ -265,3 +265,18 @@ def test_library_features_match_oracle_3d():
         np.testing.assert_allclose(X, Xo, rtol=2e-6, atol=2e-6)
         np.testing.assert_allclose(Y, Yo, rtol=2e-5, atol=2e-6)
     assert dg.closure_num_params(10, dim=3) == 64 * 32 + 64 + 64 * 64 + 64 + 10 * 64 + 10
+
+
+def test_saved_network_round_trip(tmp_path):
+    """The split pipeline's network file: FP32 master weights, the output blend and the case,
+    read back without unpickling."""
+    import closure as cl
+    case = cl.ClosureCase(dim=3, n_trunc=8, factor=2, order=3, jitter=0.15, steps=400)
+    master = dg.closure_init(cl.face_nodes(case), seed=3, out_scale=0.5, dim=3)
+    path = str(tmp_path / "net.npz")
+    cl.save_network(path, master, case, 0.25, train_runs=3)
+    m2, alpha, meta = cl.load_network(path)
+    assert np.array_equal(m2, master) and m2.dtype == np.float32 and alpha == 0.25
+    assert meta["case"]["n_trunc"] == 8 and meta["case"]["steps"] == 400 and meta["train_runs"] == 3
+    cl.save_network(path, master, case, None)
+    assert cl.load_network(path)[1] is None
