@@ -154,6 +154,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--mixed-kernel", default=None,
                     help="stage kernel of the mixed path (DGFEM_KERNEL value; default: the library's choice)")
+    ap.add_argument("--fp64-kernel", default=None,
+                    help="stage kernel of the FP64 path (DGFEM_KERNEL value; default: the library's choice)")
     ap.add_argument("--accuracy-steps", type=int, default=2000,
                     help="steps of the mixed-vs-FP64 accuracy leg (the metric's condition is at 2000)")
     args = ap.parse_args()
@@ -178,10 +180,11 @@ def main():
     states = {}
     kernels = {}
     for name, prec in (("fp64", dg.DG_FP64), ("mixed", dg.DG_MIXED)):
-        if name == "mixed" and args.mixed_kernel:
-            os.environ["DGFEM_KERNEL"] = args.mixed_kernel
+        want = args.mixed_kernel if name == "mixed" else args.fp64_kernel
+        if want:
+            os.environ["DGFEM_KERNEL"] = want
         s = make_solver(m, bc, prec, args.steps_per_graph)
-        if args.mixed_kernel:
+        if want:
             os.environ.pop("DGFEM_KERNEL", None)
         kernels[name] = dg.KERNEL_NAMES[s.info().stage_kernel]
         states[name], r, wall = accuracy_then_timed_run(s, args.accuracy_steps, args.steps, args.warmup, dt)
@@ -205,6 +208,7 @@ def main():
 
     mixed = out["mixed"]
     kernel_name = {"valu": "stageKernel<3,3,4,float,true>",
+                   "vm": "stageKernel<3,3,4,float,true,*,true,true> (volume on MFMA)",
                    "wave": "stageKernelWave<3,3,float,true>",
                    "mm": "stageKernelMM<3,3,float,true>",
                    "bf16x6": "stageKernelBf16<3,3,4,3>",

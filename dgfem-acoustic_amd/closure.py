@@ -193,17 +193,22 @@ def _solver(coords, verts, case, scen, face_bc=None, precision=dg.DG_FP64):
 
 
 def enlarged_run(case, meshes, scen, dt, steps, record_every=1, face_fraction=1.0, seed=0, min_scale=0.0,
-                 target_clip=10.0):
+                 target_clip=10.0, log=None, progress_every=0):
     """FP64 run on the enlarged mesh. Returns the observer pressure series [steps][nobs] and
     training samples (X, Y) from the truncation-face traces every record_every steps (faces
-    whose trace scale is below min_scale times the run's largest are dropped)."""
+    whose trace scale is below min_scale times the run's largest are dropped). With log and
+    progress_every, a line every progress_every steps."""
     s = _solver(meshes.parent_coords, meshes.parent_verts, case, scen)
     e, f = meshes.faces_parent
     rng = np.random.default_rng(seed)
     Xs, Ys, Ss, obs = [], [], [], []
     done = 0
+    t0 = time.perf_counter()
     while done < steps:
-        k = min(record_every, steps - done) if record_every > 0 else steps
+        if record_every > 0:
+            k = min(record_every, steps - done)
+        else:
+            k = min(progress_every, steps - done) if (log and progress_every > 0) else steps
         if record_every > 0:
             sel = np.arange(e.size) if face_fraction >= 1 else np.sort(
                 rng.choice(e.size, max(1, int(face_fraction * e.size)), replace=False))
@@ -214,6 +219,8 @@ def enlarged_run(case, meshes, scen, dt, steps, record_every=1, face_fraction=1.
             Ss.append(S)
         r = s.run_steps(k, dt, t_start=done * dt, residuals=False)
         obs.append(r.observers[:, :, 0])
+        if log and progress_every > 0 and (done + k) // progress_every > done // progress_every:
+            log(f"    enlarged run: {done + k}/{steps} steps ({time.perf_counter() - t0:.1f} s)")
         done += k
     s.close()
     if not Xs:
@@ -271,13 +278,15 @@ def scale_output(master: np.ndarray, nfp: int, alpha: float, dim: int = 2) -> np
 
 
 def collect_training(case: ClosureCase, meshes: Meshes, dt: float, steps: int, n_train: int, record_every: int = 2,
-                     face_fraction: float = 1.0, log=print):
-    """FP64 enlarged-domain training runs (seeds 1000+i) -> samples (X, Y)."""
+                     face_fraction: float = 1.0, log=print, progress_every: int = 0, min_scale: float = 0.0):
+    """FP64 enlarged-domain training runs (seeds 1000+i) -> samples (X, Y); samples whose trace
+    scale is below min_scale times the run's largest (faces the wave has not reached) are dropped."""
     Xs, Ys = [], []
     for i in range(n_train):
         scen = scenario(case, 1000 + i)
         t0 = time.perf_counter()
-        _, X, Y = enlarged_run(case, meshes, scen, dt, steps, record_every, face_fraction, seed=i)
+        _, X, Y = enlarged_run(case, meshes, scen, dt, steps, record_every, face_fraction, seed=i, log=log,
+                               progress_every=progress_every, min_scale=min_scale)
         Xs.append(X)
         Ys.append(Y)
         log(f"  training run {i + 1}/{n_train} ({scen['kind']}): {X.shape[0]} samples "
@@ -302,12 +311,12 @@ def face_nodes(case: ClosureCase) -> int:
 
 
 def choose_alpha(case: ClosureCase, meshes: Meshes, dt: float, steps: int, master, n_val: int,
-                 alphas=(0.0, 0.25, 0.5, 0.75, 1.0), log=print):
+                 alphas=(0.0, 0.25, 0.5, 0.75, 1.0), log=print, progress_every: int = 0):
     """Output blend chosen on validation runs (seeds 3000+j) by the mean observer error."""
     val = []
     for j in range(n_val):
         scen = scenario(case, 3000 + j)
-        ref, _, _ = enlarged_run(case, meshes, scen, dt, steps, record_every=0)
+        ref, _, _ = enlarged_run(case, meshes, scen, dt, steps, record_every=0, log=log, progress_every=progress_every)
         val.append((scen, ref))
     best, table = None, {}
     for a in alphas:
@@ -320,7 +329,8 @@ def choose_alpha(case: ClosureCase, meshes: Meshes, dt: float, steps: int, maste
     return best[1], table
 
 
-def evaluate(case: ClosureCase, meshes: Meshes, dt: float, steps: int, params, seeds, log=print) -> list:
+def evaluate(case: ClosureCase, meshes: Meshes, dt: float, steps: int, params, seeds, log=print,
+             progress_every: int = 0) -> list:
     """Held-out runs: the reference's characteristic BC, the exact characteristic condition
     (zero network) and the learned closure on the truncated domain, each against the
     enlarged-domain FP64 solve (observer pressure rel-L2). One dict per seed."""
@@ -328,7 +338,7 @@ def evaluate(case: ClosureCase, meshes: Meshes, dt: float, steps: int, params, s
     for j, seed in enumerate(seeds):
         scen = scenario(case, seed)
         t0 = time.perf_counter()
-        ref, _, _ = enlarged_run(case, meshes, scen, dt, steps, record_every=0)
+        ref, _, _ = enlarged_run(case, meshes, scen, dt, steps, record_every=0, log=log, progress_every=progress_every)
         t_ref = time.perf_counter() - t0
         out = dict(seed=int(seed), kind=scen["kind"], v0=[float(v) for v in scen["v0"]], enlarged_seconds=t_ref)
         for name, mode, prm in (("characteristic", "characteristic", None),
@@ -412,6 +422,9 @@ def main(argv=None):
     ap.add_argument("--alpha", type=float, default=None, help="output blend (skips the validation runs)")
     ap.add_argument("--save-network", default="", help="write the trained FP32 master weights (.npz)")
     ap.add_argument("--load-network", default="", help="skip training; use these weights (.npz)")
+    ap.add_argument("--progress-every", type=int, default=1000, help="enlarged runs: a line every N steps")
+    ap.add_argument("--min-scale", type=float, default=0.0,
+                    help="drop training samples whose trace scale is below this fraction of the run's largest")
     ap.add_argument("--out", default="", help="write the report as JSON here")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
@@ -440,22 +453,24 @@ def main(argv=None):
             alpha = saved_alpha if alpha is None else alpha
             out["network"] = meta
         else:
-            X, Y = collect_training(case, meshes, dt, steps, a.train, a.record_every, a.face_fraction, log)
+            X, Y = collect_training(case, meshes, dt, steps, a.train, a.record_every, a.face_fraction, log,
+                                    a.progress_every, a.min_scale)
             master, hist, t_train = train_network(case, X, Y, a.epochs, a.batch, a.lr, log)
             out.update(train_samples=int(X.shape[0]), epoch_loss=hist, train_seconds=t_train)
             del X, Y
         if alpha is None and a.val > 0:
-            alpha, out["validation"] = choose_alpha(case, meshes, dt, steps, master, a.val, log=log)
+            alpha, out["validation"] = choose_alpha(case, meshes, dt, steps, master, a.val, log=log,
+                                                    progress_every=a.progress_every)
         if a.save_network:
             save_network(a.save_network, master, case, alpha, train_runs=a.train, epochs=a.epochs,
-                         record_every=a.record_every, face_fraction=a.face_fraction)
+                         record_every=a.record_every, face_fraction=a.face_fraction, min_scale=a.min_scale)
             log(f"  network written to {a.save_network} (alpha {alpha})")
         out["alpha"] = alpha
         if seeds and a.eval > 0 or a.eval_seeds:
             if alpha is None:
                 raise SystemExit("held-out runs need an output blend: --alpha, --val > 0 or a saved one")
             params = dg.to_bf16(scale_output(master, face_nodes(case), alpha, case.dim))
-            out["eval"] = evaluate(case, meshes, dt, steps, params, seeds, log)
+            out["eval"] = evaluate(case, meshes, dt, steps, params, seeds, log, a.progress_every)
     if out.get("eval"):
         for k in ("characteristic", "exact_characteristic", "learned"):
             out["mean_" + k] = float(np.mean([e[k] for e in out["eval"]]))

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -53,6 +54,16 @@ struct Scope {
 };
 
 constexpr int kPer = 32;  // samples per gradient block
+static_assert(kPer == dgc::kMfmaTile, "one matrix-core tile per gradient block");
+
+// Loss and gradient kernel: the matrix-core one (default) or, with DGFEM_CLOSURE_TRAIN=valu,
+// the one-sample-per-wave VALU kernel.
+bool trainOnMatrixCores() {
+    const char* e = std::getenv("DGFEM_CLOSURE_TRAIN");
+    if (!e || !e[0] || std::string(e) == "mfma") return true;
+    if (std::string(e) == "valu") return false;
+    throw std::runtime_error(std::string("DGFEM_CLOSURE_TRAIN=") + e + " is not a training kernel (mfma, valu)");
+}
 
 template <int NIN_, int NOUT_>
 struct Net {
@@ -95,8 +106,12 @@ void lossGrad(Scope& s, const uint16_t* w, const float* X, const float* Y, const
     using N = Net<NIN, NOUT>;
     const int nparts = static_cast<int>((count + kPer - 1) / kPer);
     const float scale = 1.0f / (static_cast<float>(count) * N::NOUT);
-    hipLaunchKernelGGL((dgc::gradKernel<N::NIN, N::NOUT>), dim3(nparts), dim3(64), 0, s.stream, w, X, Y, idx, first,
-                       count, static_cast<int64_t>(kPer), scale, gpart, lpart);
+    if (trainOnMatrixCores())
+        hipLaunchKernelGGL((dgc::gradMfmaKernel<N::NIN, N::NOUT>), dim3(nparts), dim3(64), 0, s.stream, w, X, Y, idx,
+                           first, count, scale, gpart, lpart);
+    else
+        hipLaunchKernelGGL((dgc::gradKernel<N::NIN, N::NOUT>), dim3(nparts), dim3(64), 0, s.stream, w, X, Y, idx,
+                           first, count, static_cast<int64_t>(kPer), scale, gpart, lpart);
     float lr = 0, b1 = 0, b2 = 0, eps = 0, bc1 = 1, bc2 = 1;
     if (update) {
         lr = static_cast<float>(prm->lr);

@@ -30,6 +30,7 @@
 #include <cstdint>
 
 #include "dg_kernels.hpp"
+#include "dg_kernels_bf16.hpp"
 
 namespace dgc {
 
@@ -548,6 +549,315 @@ __global__ __launch_bounds__(64) void gradKernel(const uint16_t* __restrict__ w,
     for (int o = 0; o < NOUT; ++o) gp[oW3 + o * kH + lane] = g3[o];
     if (lane < NOUT) gp[oB3 + lane] = gb3;
     // wave-level loss sum (lanes >= NOUT hold 0)
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
+    if (lane == 0) lpart[blockIdx.x] = loss;
+}
+
+// ------------------------------------------------------ matrix-core training kernel ----
+//
+// The same loss and partial gradient as gradKernel for one block of kMfmaTile = 32 samples,
+// with every product of the forward and backward passes on v_mfma_f32_16x16x32_bf16 (FP32
+// accumulation): activations are matrices [unit][sample] (a 16x16 tile = 16 units x 16
+// samples), the weights (BF16 values, exact) are the A operands read straight from the BF16
+// parameters, the FP32 activations and errors are split into BF16 terms (three for the
+// forward products: FP32 operand precision; two for the backward products: ~2^-17), and the
+// weight gradients contract over the block's 32 samples (one K step):
+//   Z1 = W1 X + b1, A1 = tanh Z1; Z2 = W2 A1 + b2, A2 = tanh Z2; Y = W3 A2 + b3
+//   D3 = 2 (Y - Y*) scale;  D2 = (W3^T D3) (1 - A2^2);  D1 = (W2^T D2) (1 - A1^2)
+//   dW3 = D3 A2^T, dW2 = D2 A1^T, dW1 = D1 X^T, db = row sums of D.
+// Matrix layouts of the instruction (checked on gfx950): A lane l row l%16, k 8(l/16)+j;
+// B lane l column l%16, k 8(l/16)+j; D lane l column l%16, rows 4(l/16)+r.
+constexpr int kMfmaTile = 32;   // samples per block
+constexpr int kMfmaSS = 36;     // LDS row stride (floats) of [unit][sample] matrices
+
+// 8 BF16 weights w[(r0 + l%16) * rs + (k0 + 8(l/16) + j) * ks] (zero outside rows < R, k < K)
+// as an A fragment.
+__device__ inline dgk::u32x4_t weightFrag(const uint16_t* __restrict__ w, int lane, int r0, int rs, int R, int k0,
+                                          int ks, int K) {
+    const int row = r0 + (lane & 15), kb = k0 + 8 * (lane >> 4);
+    uint32_t h[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) h[j] = (row < R && kb + j < K) ? w[row * rs + (kb + j) * ks] : 0u;
+    return dgk::u32x4_t{h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16)};
+}
+
+// B fragment values with k along the units: m[(u0 + 8(l/16) + j) * SS + c0 + l%16].
+__device__ inline void unitFrag(const float* m, int lane, int u0, int c0, float (&v)[8]) {
+    const float* p = m + (u0 + 8 * (lane >> 4)) * kMfmaSS + c0 + (lane & 15);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = p[j * kMfmaSS];
+}
+
+// A or B fragment values with k along the samples: m[(u0 + l%16) * SS + 8(l/16) + j].
+__device__ inline void sampleFrag(const float* m, int lane, int u0, float (&v)[8]) {
+    const float4* p = reinterpret_cast<const float4*>(m + (u0 + (lane & 15)) * kMfmaSS + 8 * (lane >> 4));
+    const float4 a = p[0], b = p[1];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+
+// D = W . B for B split in NS terms (W exact): leading term and the rest in separate sums.
+template <int NS>
+__device__ inline void mmaSplitB(dgk::u32x4_t wf, const dgk::u32x4_t (&b)[NS], dgk::f32x4_t& main,
+                                 dgk::f32x4_t& corr) {
+    main = dgk::mfmaBf16(wf, b[0], main);
+#pragma unroll
+    for (int s = 1; s < NS; ++s) corr = dgk::mfmaBf16(wf, b[s], corr);
+}
+
+// Row sums over the 16 columns of a D tile (the 16 lanes of a lane group).
+__device__ inline float rowSum16(float x) {
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+template <int NIN, int NOUT>
+__global__ __launch_bounds__(64) void gradMfmaKernel(const uint16_t* __restrict__ w, const float* __restrict__ X,
+                                                     const float* __restrict__ Yt, const int64_t* __restrict__ idx,
+                                                     int64_t first, int64_t count, float scale,
+                                                     float* __restrict__ gpart, double* __restrict__ lpart) {
+    static_assert(NIN <= 32 && NOUT <= 16, "network shape of the matrix-core training kernel");
+    constexpr int P = numParams(NIN, NOUT), SS = kMfmaSS, S = kMfmaTile;
+    constexpr int oB1 = kH * NIN, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + NOUT * kH;
+    using dgk::f32x4_t;
+    using dgk::u32x4_t;
+    __shared__ __align__(16) float sX[32 * SS];    // [input][sample]
+    __shared__ __align__(16) float sA1[kH * SS];   // [hidden 1][sample]
+    __shared__ __align__(16) float sA2[kH * SS];   // [hidden 2][sample]
+    __shared__ __align__(16) float sD3[16 * SS];   // [output][sample]
+    __shared__ __align__(16) float sD2[kH * SS];
+    __shared__ __align__(16) float sD1[kH * SS];
+    __shared__ int64_t sIdx[S];
+    const int lane = threadIdx.x, g = lane >> 4, cl = lane & 15;
+    const int64_t b0 = static_cast<int64_t>(blockIdx.x) * S;
+    const int nS = static_cast<int>((count - b0) < S ? (count - b0) : S);
+    if (lane < S) sIdx[lane] = lane < nS ? (idx ? idx[first + b0 + lane] : first + b0 + lane) : -1;
+    __syncthreads();
+    // inputs (zero for padding inputs and absent samples)
+    for (int x = lane; x < 32 * S; x += 64) {
+        const int c = x / 32, i = x - c * 32;
+        const int64_t sm = sIdx[c];
+        sX[i * SS + c] = (sm >= 0 && i < NIN) ? X[sm * NIN + i] : 0.f;
+    }
+    __syncthreads();
+    const f32x4_t z4 = {0.f, 0.f, 0.f, 0.f};
+    // ---- forward: layer 1 ----
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+        const u32x4_t wf = weightFrag(w, lane, 16 * rt, NIN, kH, 0, 1, NIN);
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            float v[8];
+            u32x4_t bf[3];
+            unitFrag(sX, lane, 0, 16 * ct, v);
+            dgk::splitFrag<3>(v, bf);
+            f32x4_t m = z4, c = z4;
+            mmaSplitB<3>(wf, bf, m, c);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int h = 16 * rt + 4 * g + r;
+                sA1[h * SS + 16 * ct + cl] = tanhf(m[r] + c[r] + bf16ToFloat(w[oB1 + h]));
+            }
+        }
+    }
+    __syncthreads();
+    // ---- layer 2 ----
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+        const u32x4_t wf0 = weightFrag(w + oW2, lane, 16 * rt, kH, kH, 0, 1, kH);
+        const u32x4_t wf1 = weightFrag(w + oW2, lane, 16 * rt, kH, kH, 32, 1, kH);
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            f32x4_t m = z4, c = z4;
+            float v[8];
+            u32x4_t bf[3];
+            unitFrag(sA1, lane, 0, 16 * ct, v);
+            dgk::splitFrag<3>(v, bf);
+            mmaSplitB<3>(wf0, bf, m, c);
+            unitFrag(sA1, lane, 32, 16 * ct, v);
+            dgk::splitFrag<3>(v, bf);
+            mmaSplitB<3>(wf1, bf, m, c);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int h = 16 * rt + 4 * g + r;
+                sA2[h * SS + 16 * ct + cl] = tanhf(m[r] + c[r] + bf16ToFloat(w[oB2 + h]));
+            }
+        }
+    }
+    __syncthreads();
+    // ---- layer 3, output error, loss, b3 ----
+    double loss = 0.0;
+    float gb3[4] = {0.f, 0.f, 0.f, 0.f};
+    {
+        const u32x4_t wf0 = weightFrag(w + oW3, lane, 0, kH, NOUT, 0, 1, kH);
+        const u32x4_t wf1 = weightFrag(w + oW3, lane, 0, kH, NOUT, 32, 1, kH);
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            f32x4_t m = z4, c = z4;
+            float v[8];
+            u32x4_t bf[3];
+            unitFrag(sA2, lane, 0, 16 * ct, v);
+            dgk::splitFrag<3>(v, bf);
+            mmaSplitB<3>(wf0, bf, m, c);
+            unitFrag(sA2, lane, 32, 16 * ct, v);
+            dgk::splitFrag<3>(v, bf);
+            mmaSplitB<3>(wf1, bf, m, c);
+            const int64_t sm = sIdx[16 * ct + cl];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int o = 4 * g + r;
+                float d = 0.f;
+                if (o < NOUT && sm >= 0) {
+                    const float e = (m[r] + c[r] + bf16ToFloat(w[oB3 + o])) - Yt[sm * NOUT + o];
+                    loss += static_cast<double>(e) * static_cast<double>(e);
+                    d = 2.0f * e * scale;
+                }
+                sD3[o * SS + 16 * ct + cl] = d;
+                gb3[r] += d;
+            }
+        }
+    }
+    __syncthreads();
+    float* gp = gpart + static_cast<int64_t>(blockIdx.x) * P;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const float t = rowSum16(gb3[r]);
+        if (cl == 0 && 4 * g + r < NOUT) gp[oB3 + 4 * g + r] = t;
+    }
+    // ---- dW3 = D3 A2^T (rows: outputs, columns: hidden 2) ----
+    {
+        float v[8];
+        u32x4_t af[2];
+        sampleFrag(sD3, lane, 0, v);
+        dgk::splitFrag<2>(v, af);
+#pragma unroll
+        for (int ht = 0; ht < 4; ++ht) {
+            u32x4_t bf[2];
+            sampleFrag(sA2, lane, 16 * ht, v);
+            dgk::splitFrag<2>(v, bf);
+            f32x4_t m = dgk::mfmaBf16(af[0], bf[0], z4);
+            f32x4_t c = dgk::mfmaBf16(af[0], bf[1], z4);
+            c = dgk::mfmaBf16(af[1], bf[0], c);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int o = 4 * g + r;
+                if (o < NOUT) gp[oW3 + o * kH + 16 * ht + cl] = m[r] + c[r];
+            }
+        }
+    }
+    // ---- D2 = (W3^T D3) (1 - A2^2), b2 ----
+    {
+        // A = W3^T: rows hidden 2, k outputs (W3 is [NOUT][kH]: element (h, o) at o * kH + h)
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+            const u32x4_t wf = weightFrag(w + oW3, lane, 16 * rt, 1, kH, 0, kH, NOUT);
+            float gsum[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+                float v[8];
+                u32x4_t bf[2];
+                // k = outputs 8(l/16)+j < 32: rows of sD3 beyond 16 read as zero
+                {
+                    const int o0 = 8 * g;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) v[j] = (o0 + j < 16) ? sD3[(o0 + j) * SS + 16 * ct + cl] : 0.f;
+                }
+                dgk::splitFrag<2>(v, bf);
+                f32x4_t m = z4, c = z4;
+                mmaSplitB<2>(wf, bf, m, c);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int h = 16 * rt + 4 * g + r;
+                    const float a2 = sA2[h * SS + 16 * ct + cl];
+                    const float d = (m[r] + c[r]) * (1.0f - a2 * a2);
+                    sD2[h * SS + 16 * ct + cl] = d;
+                    gsum[r] += d;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float t = rowSum16(gsum[r]);
+                if (cl == 0) gp[oB2 + 16 * rt + 4 * g + r] = t;
+            }
+        }
+    }
+    __syncthreads();
+    // ---- dW2 = D2 A1^T ----
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+        float v[8];
+        u32x4_t af[2];
+        sampleFrag(sD2, lane, 16 * rt, v);
+        dgk::splitFrag<2>(v, af);
+#pragma unroll
+        for (int ht = 0; ht < 4; ++ht) {
+            u32x4_t bf[2];
+            sampleFrag(sA1, lane, 16 * ht, v);
+            dgk::splitFrag<2>(v, bf);
+            f32x4_t m = dgk::mfmaBf16(af[0], bf[0], z4);
+            f32x4_t c = dgk::mfmaBf16(af[0], bf[1], z4);
+            c = dgk::mfmaBf16(af[1], bf[0], c);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) gp[oW2 + (16 * rt + 4 * g + r) * kH + 16 * ht + cl] = m[r] + c[r];
+        }
+    }
+    // ---- D1 = (W2^T D2) (1 - A1^2), b1 ----
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+        // A = W2^T: rows hidden 1, k hidden 2 (element (h1, h2) of W2^T at h2 * kH + h1)
+        const u32x4_t wf0 = weightFrag(w + oW2, lane, 16 * rt, 1, kH, 0, kH, kH);
+        const u32x4_t wf1 = weightFrag(w + oW2, lane, 16 * rt, 1, kH, 32, kH, kH);
+        float gsum[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            f32x4_t m = z4, c = z4;
+            float v[8];
+            u32x4_t bf[2];
+            unitFrag(sD2, lane, 0, 16 * ct, v);
+            dgk::splitFrag<2>(v, bf);
+            mmaSplitB<2>(wf0, bf, m, c);
+            unitFrag(sD2, lane, 32, 16 * ct, v);
+            dgk::splitFrag<2>(v, bf);
+            mmaSplitB<2>(wf1, bf, m, c);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int h = 16 * rt + 4 * g + r;
+                const float a1 = sA1[h * SS + 16 * ct + cl];
+                const float d = (m[r] + c[r]) * (1.0f - a1 * a1);
+                sD1[h * SS + 16 * ct + cl] = d;
+                gsum[r] += d;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float t = rowSum16(gsum[r]);
+            if (cl == 0) gp[oB1 + 16 * rt + 4 * g + r] = t;
+        }
+    }
+    __syncthreads();
+    // ---- dW1 = D1 X^T ----
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+        float v[8];
+        u32x4_t af[2];
+        sampleFrag(sD1, lane, 16 * rt, v);
+        dgk::splitFrag<2>(v, af);
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            if (16 * it >= NIN) break;
+            u32x4_t bf[2];
+            sampleFrag(sX, lane, 16 * it, v);
+            dgk::splitFrag<2>(v, bf);
+            f32x4_t m = dgk::mfmaBf16(af[0], bf[0], z4);
+            f32x4_t c = dgk::mfmaBf16(af[0], bf[1], z4);
+            c = dgk::mfmaBf16(af[1], bf[0], c);
+            const int i = 16 * it + cl;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (i < NIN) gp[(16 * rt + 4 * g + r) * NIN + i] = m[r] + c[r];
+        }
+    }
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
     if (lane == 0) lpart[blockIdx.x] = loss;
 }

@@ -214,6 +214,7 @@ dgk::StageArgs<T> stageArgs(dg_context* c, int stage, const void* qin, void* qou
     a.code = c->code;
     a.ops = static_cast<const T*>(c->ops);
     a.iops = c->iops;
+    a.vfrag = static_cast<const T*>(c->afrag);
     a.snap = snap ? static_cast<const T*>(c->snap) : nullptr;
     a.resPartial = c->resPartial;
     a.stateBad = c->nonfinite + 1;
@@ -224,26 +225,26 @@ dgk::StageArgs<T> stageArgs(dg_context* c, int stage, const void* qin, void* qou
     return a;
 }
 
-template <int DIM, int P, int NF, typename T, bool COMP, bool WIDE>
+template <int DIM, int P, int NF, typename T, bool COMP, bool WIDE, bool VM = false>
 void launchStage(dg_context* c, int stage, const void* qin, void* qout, void* uout, bool snap) {
     using dgk::stageKernel;
     const dgk::StageArgs<T> a = stageArgs<T, COMP>(c, stage, qin, qout, uout, snap);
     const dim3 grid(static_cast<unsigned>(c->nblocks)), block(dgk::StageWaves<DIM, P, NF, T, WIDE>::kBlockSize);
     switch (dgk::stageMode(stage)) {
         case dgk::kModeFirst:
-            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeFirst, WIDE>), grid, block, 0, c->stream, a);
+            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeFirst, WIDE, VM>), grid, block, 0, c->stream, a);
             break;
         case dgk::kModeMid:
-            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeMid, WIDE>), grid, block, 0, c->stream, a);
+            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeMid, WIDE, VM>), grid, block, 0, c->stream, a);
             break;
         case dgk::kModeFinal:
-            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeFinal, WIDE>), grid, block, 0, c->stream, a);
+            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeFinal, WIDE, VM>), grid, block, 0, c->stream, a);
             break;
         case dgk::kModeEuler:
-            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeEuler, WIDE>), grid, block, 0, c->stream, a);
+            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeEuler, WIDE, VM>), grid, block, 0, c->stream, a);
             break;
         default:
-            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeRHS, WIDE>), grid, block, 0, c->stream, a);
+            hipLaunchKernelGGL((stageKernel<DIM, P, NF, T, COMP, dgk::kModeRHS, WIDE, VM>), grid, block, 0, c->stream, a);
             break;
     }
 }
@@ -252,19 +253,46 @@ void launchStage(dg_context* c, int stage, const void* qin, void* qout, void* uo
 // run 256-thread blocks, which spread the same work over twice the blocks
 constexpr int64_t kWideMinBlocks = 1024;
 
-template <int DIM, int P, int NF, typename T, bool COMP>
+template <int DIM, int P, int NF, typename T, bool COMP, bool VM = false>
 void pickStage(dg_context* c) {
     using Wd = dgk::StageWaves<DIM, P, NF, T, true>;
-    using Nw = dgk::StageWaves<DIM, P, NF, T, false>;
+    constexpr int ebWide = VM ? dgk::VmShape<DIM, P, NF, T, true>::EB : Wd::EB;
+    constexpr int ebNarrow = VM ? dgk::VmShape<DIM, P, NF, T, false>::EB : dgk::StageWaves<DIM, P, NF, T, false>::EB;
     if constexpr (Wd::kBig) {
-        if ((c->K + Wd::EB - 1) / Wd::EB < kWideMinBlocks) {
-            c->eb = Nw::EB;
-            c->stageFn = &launchStage<DIM, P, NF, T, COMP, false>;
+        if ((c->K + ebWide - 1) / ebWide < kWideMinBlocks) {
+            c->eb = ebNarrow;
+            c->stageFn = &launchStage<DIM, P, NF, T, COMP, false, VM>;
             return;
         }
     }
-    c->eb = Wd::EB;
-    c->stageFn = &launchStage<DIM, P, NF, T, COMP, true>;
+    c->eb = ebWide;
+    c->stageFn = &launchStage<DIM, P, NF, T, COMP, true, VM>;
+}
+
+// Instances of the matrix-core volume variant of the stage kernel: tetrahedra of order 2..3.
+template <int DIM, int P, int NF>
+constexpr bool vmBuilt() {
+    return DIM == 3 && (P == 2 || P == 3) && (NF == 4 || NF == 5) &&
+           dgk::VmShape<DIM, P, NF, float, true>::kSupported && dgk::VmShape<DIM, P, NF, float, false>::kSupported &&
+           dgk::VmShape<DIM, P, NF, double, true>::kSupported && dgk::VmShape<DIM, P, NF, double, false>::kSupported;
+}
+
+// A fragments [MT][KS][64 lanes] of the stacked volume operator [Dw_1; ..; Dw_DIM]: row
+// 16*mt + lane%16 = r*NP + i, k = 4*ks + lane/16 = j -> Dw_r[i][j].
+template <int DIM, int P, int NF>
+void buildVmFrag(dg_context* c, const dg::RefElement& R) {
+    using V = dgk::VmShape<DIM, P, NF, float, true>;
+    const int Np = R.Np;
+    c->afragHost.assign(V::kFrag, 0.0);
+    for (int mt = 0; mt < V::MT; ++mt)
+        for (int ks = 0; ks < V::KS; ++ks)
+            for (int lane = 0; lane < 64; ++lane) {
+                const int row = 16 * mt + lane % 16, j = 4 * ks + lane / 16;
+                if (row >= V::MR || j >= Np) continue;
+                const int r = row / Np, i = row % Np;
+                c->afragHost[(static_cast<size_t>(mt) * V::KS + ks) * 64 + lane] =
+                    R.Dw[(static_cast<size_t>(r) * Np + i) * Np + j];
+            }
 }
 
 dgk::SourceArgs sourceArgs(dg_context* c) {
@@ -548,9 +576,27 @@ bool pick(dg_context* c) {
     pickAux<DIM, P, NF>(c);
     const char* env = std::getenv("DGFEM_KERNEL");
     const std::string kind = env && env[0] ? env : defaultKernel(DIM, P, NF, c->precision);
-    if (kind != "valu" && kind != "mm" && kind != "wave" && kind != "bf16x6" && kind != "bf16x3")
+    if (kind != "valu" && kind != "mm" && kind != "wave" && kind != "bf16x6" && kind != "bf16x3" && kind != "vm")
         throw UnsupportedError("DGFEM_KERNEL=" + kind +
-                               " is not a stage-kernel variant (valu, mm, wave, bf16x6, bf16x3)");
+                               " is not a stage-kernel variant (valu, vm, mm, wave, bf16x6, bf16x3)");
+    if (kind == "vm") {
+        if constexpr (vmBuilt<DIM, P, NF>()) {
+            c->mfma = true;
+            c->kernelKind = DG_KERNEL_VM;
+            buildVmFrag<DIM, P, NF>(c, dg::refElement(DIM, P));
+            if (c->precision == DG_FP64) {
+                pickStage<DIM, P, NF, double, false, true>(c);
+                c->sourceFn = &launchSource<double, false>;
+                c->postFn = &launchPost<double, false>;
+            } else {
+                pickStage<DIM, P, NF, float, true, true>(c);
+                c->sourceFn = &launchSource<float, true>;
+                c->postFn = &launchPost<float, true>;
+            }
+            return true;
+        }
+        throw UnsupportedError("DGFEM_KERNEL=vm is built for tetrahedra of order 2 and 3 only");
+    }
     if (kind == "bf16x6" || kind == "bf16x3") {
         if (c->precision != DG_MIXED)
             throw UnsupportedError("DGFEM_KERNEL=" + kind + " is a mixed-precision kernel (FP32 state)");

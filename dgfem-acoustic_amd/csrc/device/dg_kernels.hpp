@@ -71,6 +71,7 @@ struct StageArgs {
     const uint8_t* __restrict__ code;  // [K][NFC]
     const T* __restrict__ ops;
     const int32_t* __restrict__ iops;
+    const T* __restrict__ vfrag;      // (matrix-core volume) stacked [Dw_r] A fragments [MT][KS][64]
     const T* __restrict__ snap;       // residual snapshot (leading word of the state), RK4/Euler final stage
     double* __restrict__ resPartial;  // [gridDim][5]
     int* __restrict__ stateBad;       // set to 1 by a final stage that produced a non-finite state value
@@ -270,6 +271,27 @@ __device__ inline void blockResiduals(const double (&res)[5], double (*sRed)[5],
     }
 }
 
+// Matrix-core operand types: FP32 uses v_mfma_f32_16x16x4f32 (D row = 4*(lane/16)+r), FP64
+// v_mfma_f64_16x16x4f64 (D row = lane/16 + 4*r); A lane = row + 16*k, B lane = col + 16*k
+// for both (layouts checked on gfx950).
+typedef float mf4 __attribute__((ext_vector_type(4)));
+typedef double md4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct MfmaTraits;
+template <>
+struct MfmaTraits<float> {
+    using V = mf4;
+    __device__ static V mma(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+template <>
+struct MfmaTraits<double> {
+    using V = md4;
+    __device__ static V mma(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
 template <int DIM, int P, int NF, typename T, bool WIDE = true>
 struct StageWaves {
     static constexpr int NP = Sizes<DIM, P>::NP;
@@ -281,6 +303,33 @@ struct StageWaves {
     static constexpr int kBlockSize = kBig ? 512 : kBlock;
     static constexpr int value = kBig ? (sizeof(T) == 4 ? 8 : 4) : 1;
     static constexpr int EB = kBlockSize / NP;  // elements per block
+};
+
+// Volume contraction on the matrix cores ("vm" variant of the stage kernel): the stacked
+// operator [Dw_1; ..; Dw_DIM] (DIM*NP rows x NP) times the staged state (NP rows x 16 columns =
+// ECT elements x NF fields) per 16x16 tile; each wave owns one row tile (its A fragments in
+// registers) and CPW column tiles. Dq goes node-major through LDS (aliasing the flux buffer)
+// for the flux-Jacobian fold; fluxes, lift and update stay on the VALU.
+template <int DIM, int P, int NF, typename T, bool WIDE>
+struct VmShape {
+    using W = StageWaves<DIM, P, NF, T, WIDE>;
+    static constexpr int NP = Sizes<DIM, P>::NP;
+    static constexpr int kBlockSize = W::kBlockSize;
+    static constexpr int NW = kBlockSize / 64;
+    static constexpr int ECT = 16 / NF;                       // elements per column tile
+    static constexpr int EB = (kBlockSize / NP) / ECT * ECT;  // elements per block
+    static constexpr int NCT = EB / ECT;
+    static constexpr int MR = DIM * NP;                       // stacked rows
+    static constexpr int MT = (MR + 15) / 16;                 // row tiles
+    static constexpr int KS = (NP + 3) / 4;                   // k-steps
+    static constexpr int G = NW / MT;                         // waves per row tile
+    static constexpr int CPW = G > 0 ? NCT / G : 0;           // column tiles per wave
+    static constexpr int DS = (DIM * NF + 3) / 4 * 4;         // per-node stride of staged Dq
+    // per-element stride: NP*DS + 4, so the 4 elements of a column tile start 20 (FP32) or
+    // 8 (FP64, in 32-bit words) banks apart when a row of the tile is stored
+    static constexpr int ES = NP * DS + 4;
+    static constexpr bool kSupported = G > 0 && NW % MT == 0 && NCT % G == 0 && EB > 0 && NF >= 4;
+    static constexpr int kFrag = MT * KS * 64;
 };
 
 // Padded LDS row stride (in elements) for per-element blocks of n values: == 4 (mod 16).
@@ -304,20 +353,27 @@ __host__ __device__ constexpr int stageMode(int stage) {
 // batch, the element's own stage input, inverse Jacobians, face rows, neighbour codes, and the
 // RK operands consumed at the end) -> LDS -> barrier; phase 2 (the neighbour traces of all of the
 // thread's face nodes) -> fluxes -> barrier; then the contractions and the RK update.
-template <int DIM, int P, int NF, typename T, bool COMP, int MODE, bool WIDE = true>
+template <int DIM, int P, int NF, typename T, bool COMP, int MODE, bool WIDE = true, bool VM = false>
 __global__ __launch_bounds__((StageWaves<DIM, P, NF, T, WIDE>::kBlockSize), (StageWaves<DIM, P, NF, T, WIDE>::value)) void stageKernel(StageArgs<T> a) {
     using S = Sizes<DIM, P>;
     using L = OpLayout<DIM, P>;
+    using V = VmShape<DIM, P, NF, T, WIDE>;
+    using MF = MfmaTraits<T>;
+    static_assert(!VM || V::kSupported, "matrix-core volume shape");
     constexpr int kBlock = StageWaves<DIM, P, NF, T, WIDE>::kBlockSize;  // threads per block
-    constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC, NFT = S::NFT, EB = StageWaves<DIM, P, NF, T, WIDE>::EB;
+    constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC, NFT = S::NFT;
+    constexpr int EB = VM ? V::EB : StageWaves<DIM, P, NF, T, WIDE>::EB;
     constexpr int JJ = DIM * DIM;
     constexpr int TS = static_cast<int>(sizeof(T));
-    constexpr bool kOpsInLds = (L::kTotal * TS) <= 40 * 1024;
+    // operator values staged in LDS: all of them, or only LIFT when the volume runs on MFMA
+    constexpr int kOpsOff = VM ? L::kL : 0;
+    constexpr int kOpsN = VM ? NFT * NP : L::kTotal;
+    constexpr bool kOpsInLds = (kOpsN * TS) <= 40 * 1024;
     constexpr bool kNeedU = MODE == kModeMid || MODE == kModeFinal;
     constexpr bool kFinal = MODE == kModeFinal || MODE == kModeEuler;
     // per-thread load counts
-    constexpr int kOps16 = kOpsInLds ? (L::kTotal * TS) / 16 : 0;         // 16-byte chunks
-    constexpr int kOpsRem = kOpsInLds ? L::kTotal - kOps16 * (16 / TS) : 0;  // trailing elements
+    constexpr int kOps16 = kOpsInLds ? (kOpsN * TS) / 16 : 0;         // 16-byte chunks
+    constexpr int kOpsRem = kOpsInLds ? kOpsN - kOps16 * (16 / TS) : 0;  // trailing elements
     constexpr int kOpsIt = (kOps16 + kBlock - 1) / kBlock;
     constexpr int kIIt = (L::kITotal + kBlock - 1) / kBlock;
     constexpr int kJIt = (EB * JJ + kBlock - 1) / kBlock;
@@ -330,18 +386,34 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T, WIDE>::kBlockSize), (Sta
     // per-element strides of the staged state and fluxes padded to 4 (mod 16) elements: the
     // 3-4 elements a wave spans then start in different LDS banks
     constexpr int QS = ldsStride(NP * NF), FS = ldsStride(NFT * NF);
+    constexpr int kFD = (VM && EB * V::ES > EB * FS) ? EB * V::ES : EB * FS;
     __shared__ __align__(32) T sQ[EB * QS];
-    __shared__ __align__(32) T sF[EB * FS];
+    __shared__ __align__(32) T sF[kFD];  // fluxes; with VM first the node-major Dq
     __shared__ T sJ[EB * JJ];
     __shared__ __align__(32) T sN[EB * NFC * 4];     // per (element, face): unit normal, Fscale
     __shared__ __align__(16) int32_t sNb[EB * NFC];  // per (element, face): neighbour / boundary code
     __shared__ __align__(16) uint8_t sCd8[EB * NFC]; // per (element, face): neighbour face | perm << 2
-    __shared__ __align__(16) T sOps[kOpsInLds ? L::kTotal : 1];
+    __shared__ __align__(16) T sOps[kOpsInLds ? kOpsN : 1];
     __shared__ int32_t sI[L::kITotal];
     __shared__ double sRed[kBlock / 16][5];
 
     const int tid = threadIdx.x;
-    const T* ops = kOpsInLds ? sOps : a.ops;
+    const T* ops = kOpsInLds ? sOps : a.ops;                       // (not VM) the whole table
+    const T* lift = kOpsInLds ? sOps + (L::kL - kOpsOff) : a.ops + L::kL;  // LIFT^T [fa][i]
+    // FP64 tables in LDS are held as two 32-bit planes (low words, then high words): a wave's
+    // reads of 20 consecutive doubles repeated per element hit the 32 banks twice in three of
+    // the four 16-lane passes of a 64-bit read (4.7 conflict cycles per read measured), the
+    // 32-bit plane reads none
+    constexpr bool kPlanes = kOpsInLds && sizeof(T) == 8;
+    uint32_t* const sOpsW = reinterpret_cast<uint32_t*>(sOps);
+    auto opAt = [&](int k) -> T {
+        if constexpr (kPlanes) {
+            typedef uint32_t W2 __attribute__((ext_vector_type(2)));
+            return __builtin_bit_cast(double, W2{sOpsW[k], sOpsW[kOpsN + k]});
+        } else {
+            return ops[k];
+        }
+    };
     double res[5] = {0, 0, 0, 0, 0};
 
     // one batch of EB elements per block (one batch per block measured faster than persistent
@@ -361,14 +433,21 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T, WIDE>::kBlockSize), (Sta
 #pragma unroll
         for (int c = 0; c < kOpsIt; ++c) {
             const int x = tid + c * kBlock;
-            rOps[c] = reinterpret_cast<const U4*>(a.ops)[x < kOps16 ? x : kOps16 - 1];
+            rOps[c] = reinterpret_cast<const U4*>(a.ops + kOpsOff)[x < kOps16 ? x : kOps16 - 1];
         }
-        if (kOpsRem > 0) rOpsTail = a.ops[kOps16 * (16 / TS) + (tid < kOpsRem ? tid : 0)];
+        if (kOpsRem > 0) rOpsTail = a.ops[kOpsOff + kOps16 * (16 / TS) + (tid < kOpsRem ? tid : 0)];
 #pragma unroll
         for (int c = 0; c < kIIt; ++c) {
             const int x = tid + c * kBlock;
             rI[c] = a.iops[x < L::kITotal ? x : 0];
         }
+    }
+    // (VM) this wave's row tile of the stacked operator, KS fragments
+    T rA[VM ? V::KS : 1];
+    if constexpr (VM) {
+        const int rt = (tid >> 6) % V::MT;
+#pragma unroll
+        for (int ks = 0; ks < V::KS; ++ks) rA[ks] = a.vfrag[(rt * V::KS + ks) * 64 + (tid & 63)];
     }
     T rq[NF];
     loadNode<NF>(a.qin + ldBase, rq);
@@ -410,9 +489,27 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T, WIDE>::kBlockSize), (Sta
 #pragma unroll
         for (int c = 0; c < kOpsIt; ++c) {
             const int x = tid + c * kBlock;
-            if (x < kOps16) reinterpret_cast<U4*>(sOps)[x] = rOps[c];
+            if (x < kOps16) {
+                if constexpr (kPlanes) {
+                    sOpsW[2 * x] = rOps[c].x;
+                    sOpsW[2 * x + 1] = rOps[c].z;
+                    sOpsW[kOpsN + 2 * x] = rOps[c].y;
+                    sOpsW[kOpsN + 2 * x + 1] = rOps[c].w;
+                } else {
+                    reinterpret_cast<U4*>(sOps)[x] = rOps[c];
+                }
+            }
         }
-        if (kOpsRem > 0 && tid < kOpsRem) sOps[kOps16 * (16 / TS) + tid] = rOpsTail;
+        if (kOpsRem > 0 && tid < kOpsRem) {
+            const int k = kOps16 * (16 / TS) + tid;
+            if constexpr (kPlanes) {
+                const uint64_t b = __builtin_bit_cast(uint64_t, static_cast<double>(rOpsTail));
+                sOpsW[k] = static_cast<uint32_t>(b);
+                sOpsW[kOpsN + k] = static_cast<uint32_t>(b >> 32);
+            } else {
+                sOps[k] = rOpsTail;
+            }
+        }
 #pragma unroll
         for (int c = 0; c < kIIt; ++c) {
             const int x = tid + c * kBlock;
@@ -467,6 +564,51 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T, WIDE>::kBlockSize), (Sta
     if constexpr (kFinal) {
         loadNodeNT<NF>((a.snap ? a.snap : a.qin) + ldBase, ps);
     }
+    T Dq[DIM][NF];
+#pragma unroll
+    for (int r = 0; r < DIM; ++r)
+#pragma unroll
+        for (int k = 0; k < NF; ++k) Dq[r][k] = T(0);
+    if constexpr (VM) {
+        // Dq_r = Dw_r q on the matrix cores (while the gathers are in flight), node-major into
+        // sF, then into the node threads' registers
+        const int lane = tid & 63, wave = tid >> 6;
+        const int rt = wave % V::MT, j = lane & 15, kq = lane >> 4;
+        int dOff[4];  // offsets of this lane's 4 result rows (node i, direction rd) in an element
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * rt + MF::row(lane, r);
+            const int rd = row / NP, i = row - rd * NP;
+            dOff[r] = row < V::MR ? i * V::DS + rd * NF : -1;
+        }
+#pragma unroll
+        for (int m = 0; m < V::CPW; ++m) {
+            const int ct = wave / V::MT + m * V::G;
+            const int el = ct * V::ECT + j / NF, f = j % NF;
+            const bool colOk = j < V::ECT * NF && el < nE;
+            const T* qc = &sQ[(colOk ? el : 0) * QS + f];
+            typename MF::V acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int ks = 0; ks < V::KS; ++ks) {
+                const int n = 4 * ks + kq;
+                const T b = (colOk && (4 * V::KS == NP || n < NP)) ? qc[n * NF] : T(0);
+                acc = MF::mma(rA[ks], b, acc);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (colOk && dOff[r] >= 0) sF[el * V::ES + dOff[r] + f] = acc[r];
+        }
+        __syncthreads();
+        if (active) {
+            const int el = tid / NP, i = tid - el * NP;
+            const T* d = &sF[el * V::ES + i * V::DS];
+#pragma unroll
+            for (int r = 0; r < DIM; ++r)
+#pragma unroll
+                for (int k = 0; k < NF; ++k) Dq[r][k] = d[r * NF + k];
+        }
+        __syncthreads();
+    }
 #pragma unroll
     for (int c = 0; c < kFIt; ++c) {
         const int it = tid + c * kBlock;
@@ -491,20 +633,15 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T, WIDE>::kBlockSize), (Sta
     // ---- volume contraction, flux-Jacobian fold, lift, register update ----
     if (active) {
         const int el = tid / NP, i = tid - el * NP;
-        T Dq[DIM][NF];
-#pragma unroll
-        for (int r = 0; r < DIM; ++r)
-#pragma unroll
-            for (int k = 0; k < NF; ++k) Dq[r][k] = T(0);
         const T* q0 = &sQ[el * QS];
 #pragma unroll 4
-        for (int j = 0; j < NP; ++j) {
+        for (int j = 0; j < (VM ? 0 : NP); ++j) {
             T q[NF];
 #pragma unroll
             for (int k = 0; k < NF; ++k) q[k] = q0[j * NF + k];
 #pragma unroll
             for (int r = 0; r < DIM; ++r) {
-                const T d = ops[L::kDw + (j * NP + i) * DIM + r];
+                const T d = opAt(L::kDw + (j * NP + i) * DIM + r);
 #pragma unroll
                 for (int k = 0; k < NF; ++k) Dq[r][k] += d * q[k];
             }
@@ -545,7 +682,7 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T, WIDE>::kBlockSize), (Sta
         const T* f0 = &sF[el * FS];
 #pragma unroll 4
         for (int fa = 0; fa < NFT; ++fa) {
-            const T l = ops[L::kL + fa * NP + i];
+            const T l = kPlanes ? opAt(L::kL - kOpsOff + fa * NP + i) : lift[fa * NP + i];
 #pragma unroll
             for (int k = 0; k < NF; ++k) rhs[k] -= l * f0[fa * NF + k];
         }
@@ -636,24 +773,6 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T, WIDE>::kBlockSize), (Sta
 // fields). FP32 uses v_mfma_f32_16x16x4f32 (D row = 4*(lane/16)+r), FP64
 // v_mfma_f64_16x16x4f64 (D row = lane/16 + 4*r); A lane = row + 16*k, B lane = col + 16*k
 // for both (layouts checked on gfx950).
-
-typedef float mf4 __attribute__((ext_vector_type(4)));
-typedef double md4 __attribute__((ext_vector_type(4)));
-
-template <typename T>
-struct MfmaTraits;
-template <>
-struct MfmaTraits<float> {
-    using V = mf4;
-    __device__ static V mma(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-    __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
-};
-template <>
-struct MfmaTraits<double> {
-    using V = md4;
-    __device__ static V mma(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-    __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
-};
 
 template <int DIM, int P>
 struct MfmaShape {

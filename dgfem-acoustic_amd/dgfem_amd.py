@@ -23,9 +23,9 @@ DG_FP64, DG_MIXED = 0, 1
 DG_FIELDS_REF4, DG_FIELDS_LEE5 = 0, 1
 DG_BC_ABSORBING, DG_BC_REFLECTING, DG_BC_LEARNED = 0, 1, 2
 DG_RK4, DG_EULER = 0, 1
-DG_KERNEL_VALU, DG_KERNEL_MM, DG_KERNEL_WAVE, DG_KERNEL_BF16X6, DG_KERNEL_BF16X3 = 0, 1, 2, 3, 4
+DG_KERNEL_VALU, DG_KERNEL_MM, DG_KERNEL_WAVE, DG_KERNEL_BF16X6, DG_KERNEL_BF16X3, DG_KERNEL_VM = 0, 1, 2, 3, 4, 5
 KERNEL_NAMES = {DG_KERNEL_VALU: "valu", DG_KERNEL_MM: "mm", DG_KERNEL_WAVE: "wave", DG_KERNEL_BF16X6: "bf16x6",
-                DG_KERNEL_BF16X3: "bf16x3"}
+                DG_KERNEL_BF16X3: "bf16x3", DG_KERNEL_VM: "vm"}
 
 
 class DGError(RuntimeError):

@@ -127,6 +127,29 @@ def run_closure(dim, steps, warmup=10, seed=5):
     return out
 
 
+def run_closure_train(samples=4_000_000, epochs=3, batch=8192, nfp=10, dim=3, seed=7):
+    """Closure training throughput: Adam epochs over synthetic samples of the 3D order-3 face
+    network (32 inputs, two hidden layers of 64, 10 outputs) with each training kernel."""
+    nin, nout = dg.closure_sizes(nfp, dim)
+    rng = np.random.default_rng(seed)
+    X = rng.normal(0, 1, (samples, nin)).astype(np.float32)
+    Y = (0.5 * np.tanh(X @ rng.normal(0, 0.3, (nin, nout)))).astype(np.float32)
+    out = dict(workload="closure_train", samples=samples, epochs=epochs, batch=batch, nin=nin, nout=nout,
+               describe=f"Adam on {samples} synthetic samples, network {nin}-64-64-{nout}, batch {batch}")
+    for kernel in ("mfma", "valu"):
+        os.environ["DGFEM_CLOSURE_TRAIN"] = kernel
+        m0 = dg.closure_init(nfp, seed=1, out_scale=1.0, dim=dim)
+        dg.closure_train(m0, X[:batch * 4], Y[:batch * 4], epochs=1, batch=batch, lr=1e-3, seed=2)  # warm-up
+        t0 = time.perf_counter()
+        _, _, hist = dg.closure_train(m0, X, Y, epochs=epochs, batch=batch, lr=1e-3, seed=2)
+        sec = time.perf_counter() - t0
+        out[kernel] = dict(seconds=sec, samples_per_s=samples * epochs / sec, loss=[float(v) for v in hist])
+        print(f"# {kernel}: {samples * epochs / sec:.3e} samples/s ({sec:.2f} s), loss {hist[0]:.4e} -> {hist[-1]:.4e}",
+              flush=True)
+    os.environ.pop("DGFEM_CLOSURE_TRAIN", None)
+    return out
+
+
 def run(name, steps, warmup=10, mixed_kernels=("valu",), fp64=True, **kw):
     """FP64 (unless fp64 is False) and mixed precision with each stage kernel of mixed_kernels
     (DGFEM_KERNEL values), from the same initial state; errors against FP64."""
@@ -176,14 +199,17 @@ def run(name, steps, warmup=10, mixed_kernels=("valu",), fp64=True, **kw):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["closure2d", "closure3d"], default="dense3d")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["closure2d", "closure3d", "closure_train"],
+                    default="dense3d")
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--mixed-kernels", default="valu", help="comma-separated DGFEM_KERNEL values for the mixed runs")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--n", type=int, default=None, help="box3d: hexes per side")
     ap.add_argument("--order", type=int, default=None, help="box3d: Lagrange order")
     a = ap.parse_args(argv)
-    if a.workload.startswith("closure"):
+    if a.workload == "closure_train":
+        res = run_closure_train()
+    elif a.workload.startswith("closure"):
         res = run_closure(int(a.workload[7]), a.steps)
     else:
         kw = {k: v for k, v in (("n", a.n), ("order", a.order)) if v is not None}

@@ -42,13 +42,15 @@ enum { DG_FIELDS_REF4 = 0, DG_FIELDS_LEE5 = 1 };    /* (p,vx,vy,vz) or + rho */
    face takes its exterior state from the learned closure network (dg_set_closure). */
 enum { DG_BC_ABSORBING = 0, DG_BC_REFLECTING = 1, DG_BC_LEARNED = 2 };
 enum { DG_RK4 = 0, DG_EULER = 1 };                  /* config timeIntMethod */
-/* Stage-kernel variants (dg_info.stage_kernel): the VALU kernel (default), the block and the
+/* Stage-kernel variants (dg_info.stage_kernel): the VALU kernel (default), the VALU kernel with
+   its volume contraction on the FP32/FP64 matrix cores (vm), the block and the
    wave-autonomous FP32/FP64 matrix-core kernels, and the split-BF16 matrix-core kernels of the
    mixed precision (operands as 3 BF16 terms, six products: FP32 operand precision; or 2 terms,
-   three products). Environment DGFEM_KERNEL = valu | mm | wave | bf16x6 | bf16x3; an explicit
+   three products). Environment DGFEM_KERNEL = valu | vm | mm | wave | bf16x6 | bf16x3; an explicit
    request that is not built for the mesh or precision fails dg_create with DG_ERR_UNSUPPORTED.
    Without it: bf16x6 for the mixed precision on tetrahedra of order 5, valu otherwise. */
-enum { DG_KERNEL_VALU = 0, DG_KERNEL_MM = 1, DG_KERNEL_WAVE = 2, DG_KERNEL_BF16X6 = 3, DG_KERNEL_BF16X3 = 4 };
+enum { DG_KERNEL_VALU = 0, DG_KERNEL_MM = 1, DG_KERNEL_WAVE = 2, DG_KERNEL_BF16X6 = 3, DG_KERNEL_BF16X3 = 4,
+       DG_KERNEL_VM = 5 };
 
 typedef struct dg_context dg_context;
 
@@ -222,7 +224,10 @@ DG_API dg_status dg_closure_features(int32_t dim, int32_t nfp, int32_t nf, int64
 DG_API dg_status dg_closure_forward(int32_t n_in, int32_t n_out, const uint16_t* params, const float* X, int64_t n,
                                     float* Y, int32_t device);
 /* Mean-square loss sum|y - Y|^2 / (n n_out) over all samples and its gradient grad[n_params]
-   (FP32, parameter order as above). */
+   (FP32, parameter order as above). The forward and backward products run on the BF16 matrix
+   cores in blocks of 32 samples (FP32 accumulation; activations and errors split into BF16
+   terms: three for the forward products, two for the backward ones); environment
+   DGFEM_CLOSURE_TRAIN=valu selects the one-sample-per-wave FP32 VALU kernel instead. */
 DG_API dg_status dg_closure_loss_grad(int32_t n_in, int32_t n_out, const uint16_t* params, const float* X,
                                       const float* Y, int64_t n, double* loss, float* grad, int32_t device);
 typedef struct {

@@ -36,9 +36,13 @@ def test_forward_matches_oracle(nfp, dim):
     assert np.abs(y - yo).max() <= 2e-5 * max(1.0, np.abs(yo).max())
 
 
-@pytest.mark.parametrize("nfp,n,dim", [(3, 1, 2), (3, 777, 2), (5, 5000, 2), (10, 3001, 3)])
-def test_loss_and_gradient_match_oracle(nfp, n, dim):
-    print(f"case closure grad nfp={nfp} n={n} dim={dim}", flush=True)
+@pytest.mark.parametrize("train_kernel", ["mfma", "valu"])
+@pytest.mark.parametrize("nfp,n,dim", [(3, 1, 2), (3, 777, 2), (5, 5000, 2), (6, 64, 3), (10, 3001, 3)])
+def test_loss_and_gradient_match_oracle(nfp, n, dim, train_kernel, monkeypatch):
+    """Both training kernels: the matrix-core one (split-BF16 products, the default) and the
+    one-sample-per-wave VALU one; sample counts that fill no, one or many 32-sample tiles."""
+    print(f"case closure grad nfp={nfp} n={n} dim={dim} kernel={train_kernel}", flush=True)
+    monkeypatch.setenv("DGFEM_CLOSURE_TRAIN", train_kernel)
     nin, nout = co.sizes(nfp, dim)
     bits, p64 = _params(nfp, 10 + nfp, dim=dim)
     rng = np.random.default_rng(2)
@@ -50,8 +54,10 @@ def test_loss_and_gradient_match_oracle(nfp, n, dim):
     assert rel_l2(g, go) < 1e-4, rel_l2(g, go)
 
 
-def test_training_reduces_loss_on_a_smooth_target():
-    print("case closure train synthetic", flush=True)
+@pytest.mark.parametrize("train_kernel", ["mfma", "valu"])
+def test_training_reduces_loss_on_a_smooth_target(train_kernel, monkeypatch):
+    print(f"case closure train synthetic kernel={train_kernel}", flush=True)
+    monkeypatch.setenv("DGFEM_CLOSURE_TRAIN", train_kernel)
     nfp = 5
     nin, nout = co.sizes(nfp)
     rng = np.random.default_rng(3)

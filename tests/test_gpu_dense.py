@@ -43,16 +43,18 @@ def _pulses(m, seed, nf=5):
     return u
 
 
-@pytest.mark.parametrize("p", [3, 5])
-def test_lee5_pressure_and_velocity_equal_the_four_field_oracle(p):
+@pytest.mark.parametrize("p,kernel", [(3, "valu"), (3, "vm"), (5, "valu")])
+def test_lee5_pressure_and_velocity_equal_the_four_field_oracle(p, kernel, monkeypatch):
     """3^3 x 6 tets, reflecting z=-1 + absorbing, 8 pulses, v0 = (30, -10, 5), 100 RK4 steps in
     FP64: fields (p, vx, vy, vz) of the 5-field run against the 4-field oracle, rho = p/c0^2."""
-    print(f"case lee5 vs 4-field oracle p={p}", flush=True)
+    print(f"case lee5 vs 4-field oracle p={p} kernel={kernel}", flush=True)
+    monkeypatch.setenv("DGFEM_KERNEL", kernel)
     groups = [(0x2F, dg.DG_BC_ABSORBING), (0x10, dg.DG_BC_REFLECTING)]
     v0 = (30.0, -10.0, 5.0)
     m, bc, gn = make_case(3, 3, p, groups, jitter=0.15, seed=3)
     u5 = _pulses(m, seed=11)
     s = dg.Solver(m.node_coords, m.element_vertices, p, bc, v0=v0, rho0=RHO0, c0=C0, fields=dg.DG_FIELDS_LEE5)
+    assert dg.KERNEL_NAMES[s.info().stage_kernel] == kernel
     dt = s.info().min_inradius / ((C0 + np.linalg.norm(v0)) * (2 * p + 1))
     s.set_state(u5)
     s.add_observer(0.2, -0.3, -0.8, 0.3)

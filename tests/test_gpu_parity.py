@@ -22,6 +22,16 @@ def matrix_core_built(dim, p):
     return dim == 2 or p <= 3
 
 
+def kernel_built(kernel, dim, p):
+    """valu everywhere; vm (volume on the matrix cores) on tetrahedra of order 2 and 3; mm and
+    wave on triangles and tetrahedra of order <= 3."""
+    if kernel == "valu":
+        return True
+    if kernel == "vm":
+        return dim == 3 and p in (2, 3)
+    return matrix_core_built(dim, p)
+
+
 @pytest.mark.parametrize("dim,n,p,groups,v0,c0", [
     (2, 4, 1, [(ALL2, 1)], (-30, 30, 0), 100.0),
     (2, 4, 2, [(0x3, 1), (0xC, 0)], (-30, 30, 0), 100.0),
@@ -34,14 +44,14 @@ def matrix_core_built(dim, p):
     (3, 2, 4, [(0x10, 1), (0x2F, 0)], (30, 0, 0), 340.0),
     (3, 1, 5, [(ALL3, 0)], (30, 0, 0), 340.0),
 ])
-@pytest.mark.parametrize("kernel", ["valu", "mm", "wave"])
+@pytest.mark.parametrize("kernel", ["valu", "vm", "mm", "wave"])
 def test_rhs_matches_oracle(dim, n, p, groups, v0, c0, kernel, monkeypatch):
-    """The matrix-core variants are built for triangles and for tetrahedra of order <= 3: on
-    other shapes an explicit request must fail dg_create with DG_ERR_UNSUPPORTED."""
+    """The matrix-core variants are built for some shapes only (kernel_built): on the others an
+    explicit request must fail dg_create with DG_ERR_UNSUPPORTED."""
     print(f"case rhs dim={dim} n={n} p={p} kernel={kernel}", flush=True)
     monkeypatch.setenv("DGFEM_KERNEL", kernel)
     m, bc, gn = make_case(dim, n, p, groups, seed=7)
-    if kernel != "valu" and not matrix_core_built(dim, p):
+    if not kernel_built(kernel, dim, p):
         with pytest.raises(dg.DGError) as ei:
             dg.Solver(m.node_coords, m.element_vertices, p, bc, v0=v0, rho0=RHO0, c0=c0)
         assert ei.value.code == dg.DG_ERR_UNSUPPORTED and f"DGFEM_KERNEL={kernel}" in str(ei.value)
@@ -85,7 +95,7 @@ def test_rk4_2d_reflecting_matches_oracle():
     np.testing.assert_allclose(r.observers, ref["observers"], rtol=1e-8, atol=1e-12)
 
 
-@pytest.mark.parametrize("kernel", ["valu", "mm", "wave"])
+@pytest.mark.parametrize("kernel", ["valu", "vm", "mm", "wave"])
 def test_rk4_3d_absorbing_monopole_matches_oracle(kernel, monkeypatch):
     """Reduced 3D case: 4^3 x 6 jittered Kuhn tets, p=3, v0=(30,0,0), c0=340, all
     absorbing, monopole at the origin."""
@@ -190,7 +200,7 @@ def test_euler_matches_oracle():
     np.testing.assert_allclose(r.residuals, ref["residuals"], rtol=1e-8, atol=1e-300)
 
 
-@pytest.mark.parametrize("kernel", ["valu", "mm"])
+@pytest.mark.parametrize("kernel", ["valu", "vm", "mm"])
 def test_euler_3d_with_sources_matches_oracle(kernel, monkeypatch):
     """Forward Euler on tetrahedra (p = 3, reflecting floor, absorbing elsewhere) with a monopole
     and an observer, 30 steps in batches of 7, against the oracle; the mixed-precision Euler path
@@ -241,7 +251,7 @@ def test_graph_replay_equals_eager():
     assert np.array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("kernel", ["valu", "mm", "wave"])
+@pytest.mark.parametrize("kernel", ["valu", "vm", "mm", "wave"])
 def test_mixed_precision_close_to_fp64(kernel, monkeypatch):
     """Mixed precision (FP32 state / fluxes, compensated RK4 accumulation) against GPU FP64,
     3D absorbing case with a monopole, 500 steps: pressure rel L2 <= 1e-6. Snapshots every
