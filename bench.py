@@ -208,7 +208,7 @@ def main():
 
     mixed = out["mixed"]
     kernel_name = {"valu": "stageKernel<3,3,4,float,true>",
-                   "vm": "stageKernel<3,3,4,float,true,*,true,true> (volume on MFMA)",
+                   "vm": "stageKernel<3,3,4,float,true,VM>",
                    "wave": "stageKernelWave<3,3,float,true>",
                    "mm": "stageKernelMM<3,3,float,true>",
                    "bf16x6": "stageKernelBf16<3,3,4,3>",

@@ -420,6 +420,7 @@ def main(argv=None):
     ap.add_argument("--face-fraction", type=float, default=1.0)
     ap.add_argument("--observer-size", type=float, default=0.03)
     ap.add_argument("--alpha", type=float, default=None, help="output blend (skips the validation runs)")
+    ap.add_argument("--alphas", default="0,0.25,0.5,0.75,1", help="output blends tried on the validation runs")
     ap.add_argument("--save-network", default="", help="write the trained FP32 master weights (.npz)")
     ap.add_argument("--load-network", default="", help="skip training; use these weights (.npz)")
     ap.add_argument("--progress-every", type=int, default=1000, help="enlarged runs: a line every N steps")
@@ -459,7 +460,8 @@ def main(argv=None):
             out.update(train_samples=int(X.shape[0]), epoch_loss=hist, train_seconds=t_train)
             del X, Y
         if alpha is None and a.val > 0:
-            alpha, out["validation"] = choose_alpha(case, meshes, dt, steps, master, a.val, log=log,
+            alpha, out["validation"] = choose_alpha(case, meshes, dt, steps, master, a.val,
+                                                    tuple(float(v) for v in a.alphas.split(",")), log=log,
                                                     progress_every=a.progress_every)
         if a.save_network:
             save_network(a.save_network, master, case, alpha, train_runs=a.train, epochs=a.epochs,

@@ -121,7 +121,8 @@ void lossGrad(Scope& s, const uint16_t* w, const float* X, const float* Y, const
         bc1 = static_cast<float>(1.0 - std::pow(prm->beta1, static_cast<double>(step)));
         bc2 = static_cast<float>(1.0 - std::pow(prm->beta2, static_cast<double>(step)));
     }
-    hipLaunchKernelGGL(dgc::reduceAdamKernel, dim3((N::P + 255) / 256), dim3(256), 0, s.stream, gpart, nparts, N::P,
+    hipLaunchKernelGGL(dgc::reduceAdamKernel, dim3((N::P + dgc::kReduceParams - 1) / dgc::kReduceParams), dim3(256), 0,
+                       s.stream, gpart, nparts, N::P,
                        grad, master, m1, m2, wbf, lr, b1, b2, eps, bc1, bc2, update ? 1 : 0);
     CHK(hipGetLastError());
     if (lossHost) {

@@ -862,17 +862,29 @@ __global__ __launch_bounds__(64) void gradMfmaKernel(const uint16_t* __restrict_
     if (lane == 0) lpart[blockIdx.x] = loss;
 }
 
-// Sum of the block partials (in block order) -> grad, and optionally one Adam step on the
-// FP32 master parameters with a BF16 copy for the next forward pass.
+// Sum of the block partials -> grad, and optionally one Adam step on the FP32 master
+// parameters with a BF16 copy for the next forward pass. A 256-thread block takes 16
+// parameters; its 16 slices of threads sum the partials b = slice, slice + 16, ... in
+// ascending order, then the 16 slice sums are added in slice order (a fixed order: the result
+// does not depend on timing).
+constexpr int kReduceParams = 16, kReduceSlices = 16;
 [[maybe_unused]] static __global__ __launch_bounds__(256) void reduceAdamKernel(const float* __restrict__ gpart, int nparts, int P,
                                                         float* __restrict__ grad, float* __restrict__ master,
                                                         float* __restrict__ m1, float* __restrict__ m2,
                                                         uint16_t* __restrict__ wbf, float lr, float beta1, float beta2,
                                                         float eps, float bc1, float bc2, int doUpdate) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= P) return;
+    __shared__ float part[kReduceSlices][kReduceParams];
+    const int pl = threadIdx.x % kReduceParams, slice = threadIdx.x / kReduceParams;
+    const int x = blockIdx.x * kReduceParams + pl;
     float g = 0.f;
-    for (int b = 0; b < nparts; ++b) g += gpart[static_cast<int64_t>(b) * P + x];
+    if (x < P)
+        for (int b = slice; b < nparts; b += kReduceSlices) g += gpart[static_cast<int64_t>(b) * P + x];
+    part[slice][pl] = g;
+    __syncthreads();
+    if (slice != 0 || x >= P) return;
+    g = 0.f;
+#pragma unroll
+    for (int k = 0; k < kReduceSlices; ++k) g += part[k][pl];
     if (grad) grad[x] = g;
     if (!doUpdate) return;
     const float a = beta1 * m1[x] + (1.f - beta1) * g;

@@ -543,12 +543,15 @@ void pickAux(dg_context* c) {
 
 // Stage kernel without DGFEM_KERNEL: the split-BF16 matrix-core kernel for the mixed precision
 // where it measured faster than the VALU kernel (tetrahedra of order 5: 5.07 against 7.77 ms per
-// step on the 511,104-tet 5-field case, at lower error against FP64); the VALU kernel elsewhere
-// (order 3: 0.62 against 0.40 ms per step on the 196,608-tet 4-field case, whose contractions are
+// step on the 511,104-tet 5-field case, at lower error against FP64); on tetrahedra of order 3
+// the VALU kernel with its volume contraction on the matrix cores (0.5-4% faster per step than
+// the VALU kernel in both precisions on the 196,608-tet case, on three boxes); the VALU kernel
+// elsewhere (the split-BF16 kernel at order 3: 0.62 against 0.40 ms per step, its contractions
 // too small a part of a stage for the split operands to pay).
 std::string defaultKernel(int dim, int p, int nf, int precision) {
     if (precision == DG_MIXED && dim == 3 && p >= kBf16DefaultMinOrder && dgk::stageBf16Built(dim, p, nf, 3))
         return "bf16x6";
+    if (dim == 3 && p == 3 && (nf == 4 || nf == 5)) return "vm";
     return "valu";
 }
 

@@ -390,7 +390,15 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T, WIDE>::kBlockSize), (Sta
     __shared__ __align__(32) T sQ[EB * QS];
     __shared__ __align__(32) T sF[kFD];  // fluxes; with VM first the node-major Dq
     __shared__ T sJ[EB * JJ];
-    __shared__ __align__(32) T sN[EB * NFC * 4];     // per (element, face): unit normal, Fscale
+    // per (element, face): unit normal n, Fscale fs, then (NSV = 8) h n and h (v0.n) with
+    // h = fs / 2 (the interior flux with its face scale folded in) where those 4 values per face
+    // leave the blocks per CU that the LDS allows unchanged
+    constexpr auto ldsBytes = [](int nsv) {
+        return TS * (EB * QS + kFD + EB * JJ + EB * NFC * nsv + (kOpsInLds ? kOpsN : 1)) + 5 * EB * NFC +
+               4 * L::kITotal + 8 * 5 * (kBlock / 16);
+    };
+    constexpr int NSV = (160 * 1024) / ldsBytes(8) >= (160 * 1024) / ldsBytes(4) ? 8 : 4;
+    __shared__ __align__(32) T sN[EB * NFC * NSV];
     __shared__ __align__(16) int32_t sNb[EB * NFC];  // per (element, face): neighbour / boundary code
     __shared__ __align__(16) uint8_t sCd8[EB * NFC]; // per (element, face): neighbour face | perm << 2
     __shared__ __align__(16) T sOps[kOpsInLds ? kOpsN : 1];
@@ -525,7 +533,20 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T, WIDE>::kBlockSize), (Sta
         const int x = tid + c * kBlock;
         if (x < nE * JJ) sJ[x] = rJ[c];
     }
-    if (faceRow) reinterpret_cast<V4*>(sN)[tid] = rN;
+    if (faceRow) {
+        if constexpr (NSV == 8) {
+            const T h = T(0.5) * rN.w;
+            V4 hn;
+            hn.x = h * rN.x;
+            hn.y = h * rN.y;
+            hn.z = h * rN.z;
+            hn.w = a.ph.v0[0] * hn.x + a.ph.v0[1] * hn.y + a.ph.v0[2] * hn.z;
+            reinterpret_cast<V4*>(sN)[2 * tid] = rN;
+            reinterpret_cast<V4*>(sN)[2 * tid + 1] = hn;
+        } else {
+            reinterpret_cast<V4*>(sN)[tid] = rN;
+        }
+    }
     if constexpr (NFC == 4) {
         if (tid < nE) {
             reinterpret_cast<U4*>(sNb)[tid] = rNb4;
@@ -616,16 +637,38 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T, WIDE>::kBlockSize), (Sta
             const int el = it / NFT, fa = it - el * NFT;
             const int f = fa / NFP, an = fa - f * NFP;
             const int efl = el * NFC + f;
-            const T* nf = &sN[efl * 4];
-            const T nv[3] = {nf[0], nf[1], nf[2]};
-            const T fs = nf[3];
+            const T* nf = &sN[efl * NSV];
             const int nb = sNb[efl];
             const int own = sI[L::kMask + f * NFP + an];
             T qm[NF], pi[NF];
             loadNode<NF>(&sQ[el * QS + own * NF], qm);
-            numericalFlux<DIM, NF>(a.ph, nv, nb <= -3 ? 0 : nb, qm, qp[c], pi);
+            if (NSV == 8 && (nb >= 0 || nb <= -3)) {
+                // interior (and learned-closure) faces, scaled by fs:
+                // (fs/2) n.F(q- + q+) + fs (c0/2)(q- - q+), F linear in q
+                const T hn[3] = {nf[4], nf[5], nf[6]};
+                const T hv = nf[7], cf = nf[3] * (T(0.5) * a.ph.c0);
+                const T* qp0 = qp[c];
+                T qs[NF], qd[NF];
 #pragma unroll
-            for (int k = 0; k < NF; ++k) sF[el * FS + fa * NF + k] = fs * pi[k];
+                for (int k = 0; k < NF; ++k) {
+                    qs[k] = qm[k] + qp0[k];
+                    qd[k] = qm[k] - qp0[k];
+                }
+                const T nvs = hn[0] * qs[1] + hn[1] * qs[2] + hn[2] * qs[3];
+                const T t = a.ph.irho0 * qs[0];
+                pi[0] = hv * qs[0] + a.ph.rc2 * nvs + cf * qd[0];
+#pragma unroll
+                for (int x = 0; x < 3; ++x) pi[1 + x] = hv * qs[1 + x] + hn[x] * t + cf * qd[1 + x];
+                if constexpr (NF == 5) pi[4] = hv * qs[4] + a.ph.rho0 * nvs + cf * qd[4];
+            } else {
+                const T nv[3] = {nf[0], nf[1], nf[2]};
+                const T fs = nf[3];
+                numericalFlux<DIM, NF>(a.ph, nv, nb <= -3 ? 0 : nb, qm, qp[c], pi);
+#pragma unroll
+                for (int k = 0; k < NF; ++k) pi[k] *= fs;
+            }
+#pragma unroll
+            for (int k = 0; k < NF; ++k) sF[el * FS + fa * NF + k] = pi[k];
         }
     }
     __syncthreads();
@@ -646,38 +689,42 @@ __global__ __launch_bounds__((StageWaves<DIM, P, NF, T, WIDE>::kBlockSize), (Sta
                 for (int k = 0; k < NF; ++k) Dq[r][k] += d * q[k];
             }
         }
-        // W_x = sum_r (dxi_r/dx_x) Dq_r ; rhs = sum_x A_x W_x
+        // rhs = sum_x A_x W_x with W_x = sum_r (dxi_r/dx_x) Dq_r, A_x the flux Jacobians, in the
+        // order sum_r B_r Dq_r: a_r = sum_x (dxi_r/dx_x) v0_x multiplies every field, the
+        // pressure-velocity coupling enters through sdiv = sum_r sum_x (dxi_r/dx_x) Dq_r[v_x]
+        // and sum_r (dxi_r/dx_k) Dq_r[p]
         const T* J = &sJ[el * JJ];
-        T W[3][NF];
+        T ar[DIM];
 #pragma unroll
-        for (int x = 0; x < 3; ++x)
+        for (int r = 0; r < DIM; ++r) {
+            T s = T(0);
+#pragma unroll
+            for (int x = 0; x < DIM; ++x) s += J[r * DIM + x] * a.ph.v0[x];
+            ar[r] = s;
+        }
+        T rhs[NF];
+        {
+            T sdiv = T(0);
+#pragma unroll
+            for (int r = 0; r < DIM; ++r)
+#pragma unroll
+                for (int x = 0; x < DIM; ++x) sdiv += J[r * DIM + x] * Dq[r][1 + x];
 #pragma unroll
             for (int k = 0; k < NF; ++k) {
                 T s = T(0);
-                if (x < DIM) {
 #pragma unroll
-                    for (int r = 0; r < DIM; ++r) s += J[r * DIM + x] * Dq[r][k];
-                }
-                W[x][k] = s;
+                for (int r = 0; r < DIM; ++r) s += ar[r] * Dq[r][k];
+                rhs[k] = s;
             }
-        T rhs[NF];
-        {
-            T rp = T(0), rr = T(0);
+            rhs[0] += a.ph.rc2 * sdiv;
+            if constexpr (NF == 5) rhs[4] += a.ph.rho0 * sdiv;
 #pragma unroll
-            for (int x = 0; x < DIM; ++x) {
-                rp += a.ph.v0[x] * W[x][0] + a.ph.rc2 * W[x][1 + x];
-                if constexpr (NF == 5) rr += a.ph.v0[x] * W[x][4] + a.ph.rho0 * W[x][1 + x];
+            for (int kv = 0; kv < DIM; ++kv) {
+                T g = T(0);
+#pragma unroll
+                for (int r = 0; r < DIM; ++r) g += J[r * DIM + kv] * Dq[r][0];
+                rhs[1 + kv] += a.ph.irho0 * g;
             }
-            rhs[0] = rp;
-#pragma unroll
-            for (int kv = 0; kv < 3; ++kv) {
-                T s = T(0);
-#pragma unroll
-                for (int x = 0; x < DIM; ++x) s += a.ph.v0[x] * W[x][1 + kv];
-                if (kv < DIM) s += W[kv][0] * a.ph.irho0;
-                rhs[1 + kv] = s;
-            }
-            if constexpr (NF == 5) rhs[4] = rr;
         }
         const T* f0 = &sF[el * FS];
 #pragma unroll 4

@@ -257,7 +257,7 @@ __global__ __launch_bounds__(256, (bf16BlocksPerCu<DIM, P, NF, NSPLIT>())) void 
     using L = OpLayout<DIM, P>;
     using B = Bf16Shape<DIM, P, NF>;
     constexpr int NP = S::NP, NFP = S::NFP, NFC = S::NFC, NFT = S::NFT;
-    constexpr int MT = B::MT, CPR = B::CPR, KSV = B::KSV, KS = B::KS, NCL = B::NCL, G = B::kGroup;
+    constexpr int MT = B::MT, KSV = B::KSV, KS = B::KS, G = B::kGroup;
     constexpr int QS = B::QS, QE = B::QE, PR = B::PR, PE = B::PE, BS = B::kBlockSize;
     constexpr int JJ = DIM * DIM;
     constexpr int WPT = B::WPT, FPW = B::FPW;
@@ -460,7 +460,8 @@ __global__ __launch_bounds__(256, (bf16BlocksPerCu<DIM, P, NF, NSPLIT>())) void 
 #pragma unroll
             for (int s = 0; s < NSPLIT; ++s) anext[s] = aw[(k0 * NSPLIT + s) * 64];
         }
-#pragma unroll (kAStream ? 1 : KS)
+    constexpr int kUnrollK = kAStream ? 1 : KS;
+#pragma unroll kUnrollK
         for (int ks = k0; ks < k1; ++ks) {
             u32x4_t acur[NSPLIT];
             if constexpr (kAStream) {

@@ -150,14 +150,15 @@ def run_closure_train(samples=4_000_000, epochs=3, batch=8192, nfp=10, dim=3, se
     return out
 
 
-def run(name, steps, warmup=10, mixed_kernels=("valu",), fp64=True, **kw):
-    """FP64 (unless fp64 is False) and mixed precision with each stage kernel of mixed_kernels
-    (DGFEM_KERNEL values), from the same initial state; errors against FP64."""
+def run(name, steps, warmup=10, mixed_kernels=("valu",), fp64=True, fp64_kernel="valu", **kw):
+    """FP64 (unless fp64 is False; stage kernel fp64_kernel) and mixed precision with each
+    stage kernel of mixed_kernels (DGFEM_KERNEL values), from the same initial state; errors
+    against FP64."""
     w = WORKLOADS[name](**kw)
     m = w["mesh"]
     out = dict(workload=name, describe=w["describe"], steps=steps)
     states = {}
-    runs = ([("fp64", dg.DG_FP64, "valu")] if fp64 else []) + [("mixed_" + k, dg.DG_MIXED, k) for k in mixed_kernels]
+    runs = ([("fp64", dg.DG_FP64, fp64_kernel)] if fp64 else []) + [("mixed_" + k, dg.DG_MIXED, k) for k in mixed_kernels]
     for label, prec, kernel in runs:
         os.environ["DGFEM_KERNEL"] = kernel
         s = dg.Solver(m.node_coords, m.element_vertices, w["order"], w["bc"], v0=w["v0"], rho0=RHO0, c0=w["c0"],
@@ -204,6 +205,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--mixed-kernels", default="valu", help="comma-separated DGFEM_KERNEL values for the mixed runs")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--fp64-kernel", default="valu", help="DGFEM_KERNEL value of the FP64 run")
     ap.add_argument("--n", type=int, default=None, help="box3d: hexes per side")
     ap.add_argument("--order", type=int, default=None, help="box3d: Lagrange order")
     a = ap.parse_args(argv)
@@ -213,7 +215,8 @@ def main(argv=None):
         res = run_closure(int(a.workload[7]), a.steps)
     else:
         kw = {k: v for k, v in (("n", a.n), ("order", a.order)) if v is not None}
-        res = run(a.workload, a.steps, mixed_kernels=tuple(a.mixed_kernels.split(",")), fp64=not a.no_fp64, **kw)
+        res = run(a.workload, a.steps, mixed_kernels=tuple(a.mixed_kernels.split(",")), fp64=not a.no_fp64,
+                  fp64_kernel=a.fp64_kernel, **kw)
     print(json.dumps(res), flush=True)
 
 

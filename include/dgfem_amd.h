@@ -48,7 +48,8 @@ enum { DG_RK4 = 0, DG_EULER = 1 };                  /* config timeIntMethod */
    mixed precision (operands as 3 BF16 terms, six products: FP32 operand precision; or 2 terms,
    three products). Environment DGFEM_KERNEL = valu | vm | mm | wave | bf16x6 | bf16x3; an explicit
    request that is not built for the mesh or precision fails dg_create with DG_ERR_UNSUPPORTED.
-   Without it: bf16x6 for the mixed precision on tetrahedra of order 5, valu otherwise. */
+   Without it: bf16x6 for the mixed precision on tetrahedra of order 5, vm on tetrahedra of
+   order 3, valu otherwise. */
 enum { DG_KERNEL_VALU = 0, DG_KERNEL_MM = 1, DG_KERNEL_WAVE = 2, DG_KERNEL_BF16X6 = 3, DG_KERNEL_BF16X3 = 4,
        DG_KERNEL_VM = 5 };
 
